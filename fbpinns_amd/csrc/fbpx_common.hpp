@@ -1,0 +1,176 @@
+// Shared device/host helpers for libfbpx (gfx950).
+//
+// Jet layout (per point or per pair): J floats
+//   [value, d/dx_i for i in D1 (ascending), d2/dx_i^2 for i in D2 (ascending)]
+// This is the set of derivatives that the reference's nested jvp tree
+// (trainers.py:63-107, 213-247) produces for required_ujs of order <= 2 with
+// repeated indices; it is carried forward through every layer at once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fbpx.h"
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+namespace fbpx {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+__host__ __device__ constexpr int popc(int v) { return v == 0 ? 0 : (v & 1) + popc(v >> 1); }
+
+template <int XD_, int D1M_, int D2M_>
+struct Schema {
+  static constexpr int XD = XD_;
+  static constexpr int D1M = D1M_;
+  static constexpr int D2M = D2M_;
+  static constexpr int N1 = popc(D1M);
+  static constexpr int N2 = popc(D2M);
+  static constexpr int J = 1 + N1 + N2;
+  static_assert((D2M & ~D1M) == 0, "second-order dims must carry first order");
+  static_assert(D1M < (1 << XD), "mask exceeds dimension");
+  __host__ __device__ static constexpr bool has1(int i) { return (D1M >> i) & 1; }
+  __host__ __device__ static constexpr bool has2(int i) { return (D2M >> i) & 1; }
+  __host__ __device__ static constexpr int col1(int i) { return 1 + popc(D1M & ((1 << i) - 1)); }
+  __host__ __device__ static constexpr int col2(int i) { return 1 + N1 + popc(D2M & ((1 << i) - 1)); }
+};
+
+// Offsets of one subdomain's parameter row; layer sizes [XD, H x NH, 1],
+// W stored (out, in) row-major as in networks.py:50-58.
+template <int XD, int H, int NH>
+struct Layout {
+  static constexpr int W1 = 0;
+  static constexpr int B1 = W1 + H * XD;
+  static constexpr int W2 = B1 + H;
+  static constexpr int B2 = W2 + (NH == 2 ? H * H : 0);
+  static constexpr int WL = B2 + (NH == 2 ? H : 0);
+  static constexpr int BL = WL + H;
+  static constexpr int P = BL + 1;
+};
+
+__host__ __device__ inline int param_count(int xd, int h, int nh) {
+  return h * xd + h + (nh == 2 ? h * h + h : 0) + h + 1;
+}
+
+// Static row of one subdomain (decompositions.py:127-128,181).
+struct Dom {
+  float xmin[FBPX_MAX_XD], xmax[FBPX_MAX_XD];
+  float umu, usd, flag;
+};
+
+__device__ inline void load_dom(const float* dom, int model, Dom& d) {
+  const float* r = dom + (size_t)model * FBPX_DOM_STRIDE;
+#pragma unroll
+  for (int i = 0; i < FBPX_MAX_XD; ++i) {
+    d.xmin[i] = r[i];
+    d.xmax[i] = r[4 + i];
+  }
+  d.umu = r[8];
+  d.usd = r[9];
+  d.flag = r[10];
+}
+
+// tanh from the identity tanh(z) = sign(z) (1 - e) / (1 + e), e = exp(-2|z|),
+// with an odd Taylor polynomial for |z| < 0.25 to avoid the cancellation in
+// 1 - e (relative error < 4e-7 on both branches).
+__device__ inline float tanh_f(float z) {
+  float a = fabsf(z);
+  float t;
+  if (a < 0.25f) {
+    float z2 = z * z;
+    // z - z^3/3 + 2z^5/15 - 17z^7/315 + 62z^9/2835
+    float p = fmaf(z2, 62.f / 2835.f, -17.f / 315.f);
+    p = fmaf(z2, p, 2.f / 15.f);
+    p = fmaf(z2, p, -1.f / 3.f);
+    t = fmaf(z * z2, p, z);
+    return t;
+  }
+  float e = __expf(-2.f * a);
+  t = (1.f - e) / (1.f + e);
+  return copysignf(t, z);
+}
+
+__device__ inline f4 mfma16(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Window jets of one pair (windows.py:25-35 with decompositions.py:196-199).
+// Pairs only exist for points inside the box (inclusive), where both
+// heaviside factors equal 1 and have zero derivative.
+template <class S>
+__device__ inline void window_jets(const float* x, const Dom& d, float* w) {
+  constexpr int XD = S::XD;
+  float c[XD], c1[XD], c2[XD];
+#pragma unroll
+  for (int i = 0; i < XD; ++i) {
+    float mu = (d.xmin[i] + d.xmax[i]) / 2.f;
+    float sd = (d.xmax[i] - d.xmin[i]) / 2.f;
+    float th = kPi * (x[i] - mu) / sd;
+    float s, co;
+    sincosf(th, &s, &co);
+    float hc = (1.f + co) / 2.f;
+    float k = kPi / sd;
+    c[i] = hc * hc;
+    c1[i] = -k * (1.f + co) * s * 0.5f;
+    c2[i] = k * k * (s * s - co * (1.f + co)) * 0.5f;
+  }
+  float prod = 1.f;
+#pragma unroll
+  for (int i = 0; i < XD; ++i) prod *= c[i];
+  w[0] = d.flag * prod + (1.f - d.flag);
+#pragma unroll
+  for (int i = 0; i < XD; ++i) {
+    float others = 1.f;
+#pragma unroll
+    for (int j = 0; j < XD; ++j)
+      if (j != i) others *= c[j];
+    if (S::has1(i)) w[S::col1(i)] = d.flag * c1[i] * others;
+    if (S::has2(i)) w[S::col2(i)] = d.flag * c2[i] * others;
+  }
+}
+
+// Leibniz product of two jets (uw = u * w), diagonal second order.
+template <class S>
+__device__ inline void jet_mul(const float* u, const float* w, float* out) {
+  out[0] = u[0] * w[0];
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (S::has1(i)) {
+      const int a = S::col1(i);
+      out[a] = u[a] * w[0] + u[0] * w[a];
+      if (S::has2(i)) {
+        const int b = S::col2(i);
+        out[b] = u[b] * w[0] + 2.f * u[a] * w[a] + u[0] * w[b];
+      }
+    }
+  }
+}
+
+// Transpose of jet_mul w.r.t. u: given g = dL/d(uw), returns dL/du.
+template <class S>
+__device__ inline void jet_mul_T(const float* g, const float* w, float* gu) {
+  float g0 = g[0] * w[0];
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (S::has1(i)) {
+      const int a = S::col1(i);
+      g0 += g[a] * w[a];
+      float ga = g[a] * w[0];
+      if (S::has2(i)) {
+        const int b = S::col2(i);
+        g0 += g[b] * w[b];
+        ga += 2.f * g[b] * w[a];
+        gu[b] = g[b] * w[0];
+      }
+      gu[a] = ga;
+    }
+  }
+  gu[0] = g0;
+}
+
+// Thread-local error text; defined in fbpx_host.hip.
+void set_error(const char* fmt, ...);
+int hip_check(hipError_t e, const char* what);
+
+}  // namespace fbpx

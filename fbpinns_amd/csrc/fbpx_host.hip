@@ -1,0 +1,111 @@
+// C-ABI entry points for the pair kernels, error reporting and shape queries.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "fbpx_shapes.hpp"
+
+namespace fbpx {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return FBPX_OK;
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return FBPX_EHIP;
+}
+
+static int check_net(const fbpx_net* net) {
+  if (!net) {
+    set_error("null net");
+    return FBPX_EINVAL;
+  }
+  if (net->xd < 1 || net->xd > FBPX_MAX_XD || (net->d2mask & ~net->d1mask) || net->d1mask >= (1 << net->xd) ||
+      net->d1mask < 0 || net->d2mask < 0) {
+    set_error("bad net xd=%d d1mask=%d d2mask=%d", net->xd, net->d1mask, net->d2mask);
+    return FBPX_EINVAL;
+  }
+  if (!(net->h == 16 || net->h == 32 || net->h == 64) || !(net->nh == 1 || net->nh == 2)) {
+    set_error("network [%d, %d x %d, 1] not compiled (hidden width 16/32/64, 1 or 2 hidden layers)", net->xd,
+              net->h, net->nh);
+    return FBPX_EUNSUPPORTED;
+  }
+  if (!schema_supported(net->xd, net->d1mask, net->d2mask)) {
+    set_error("jet schema xd=%d d1mask=%d d2mask=%d not compiled", net->xd, net->d1mask, net->d2mask);
+    return FBPX_EUNSUPPORTED;
+  }
+  return FBPX_OK;
+}
+
+static int check_pair_args(const fbpx_net* net, const float* theta, const float* dom, const float* x,
+                           const fbpx_worklist* wl, const void* out) {
+  int rc = check_net(net);
+  if (rc) return rc;
+  if (!wl || wl->n_pairs < 0 || wl->n_chunks < 0) {
+    set_error("bad work list");
+    return FBPX_EINVAL;
+  }
+  if (wl->n_chunks > 0 && (!theta || !dom || !x || !out || !wl->chunk_model || !wl->chunk_start ||
+                           !wl->pair_point)) {
+    set_error("null pointer in pair kernel arguments");
+    return FBPX_EINVAL;
+  }
+  return FBPX_OK;
+}
+
+typedef int (*launch_fn)(const fbpx_net&, const NetArgs&, hipStream_t);
+
+static int dispatch(const launch_fn* fns, const fbpx_net* net, const NetArgs& a, hipStream_t s) {
+  const int rc = fns[net->xd - 1](*net, a, s);
+  if (rc == FBPX_EUNSUPPORTED)
+    set_error("no kernel for xd=%d h=%d nh=%d d1mask=%d d2mask=%d", net->xd, net->h, net->nh, net->d1mask,
+              net->d2mask);
+  return rc;
+}
+
+}  // namespace fbpx
+
+using namespace fbpx;
+
+extern "C" {
+
+int fbpx_forward(void* stream, const fbpx_net* net, const float* theta, const float* dom, const float* x,
+                 const fbpx_worklist* wl, float* pairbuf) {
+  int rc = check_pair_args(net, theta, dom, x, wl, pairbuf);
+  if (rc) return rc;
+  static const launch_fn fns[3] = {launch_fwd_xd1, launch_fwd_xd2, launch_fwd_xd3};
+  NetArgs a{theta, dom, x, *wl, nullptr, nullptr, pairbuf};
+  return dispatch(fns, net, a, (hipStream_t)stream);
+}
+
+int fbpx_backward(void* stream, const fbpx_net* net, const float* theta, const float* dom, const float* x,
+                  const fbpx_worklist* wl, const uint8_t* chunk_train, const float* gseg, float* partial) {
+  int rc = check_pair_args(net, theta, dom, x, wl, partial);
+  if (rc) return rc;
+  if (wl->n_chunks > 0 && (!gseg || !wl->pair_seg)) {
+    set_error("backward: null gseg / pair_seg");
+    return FBPX_EINVAL;
+  }
+  static const launch_fn fns[3] = {launch_bwd_xd1, launch_bwd_xd2, launch_bwd_xd3};
+  NetArgs a{theta, dom, x, *wl, chunk_train, gseg, partial};
+  return dispatch(fns, net, a, (hipStream_t)stream);
+}
+
+int32_t fbpx_param_count(const fbpx_net* net) {
+  if (!net || net->xd < 1 || net->h < 1 || net->nh < 1 || net->nh > 2) return 0;
+  return param_count(net->xd, net->h, net->nh);
+}
+
+int32_t fbpx_supported(const fbpx_net* net) { return check_net(net) == FBPX_OK ? 1 : 0; }
+
+const char* fbpx_last_error(void) { return g_err; }
+
+const char* fbpx_version(void) { return "fbpx 0.1 gfx950"; }
+
+}  // extern "C"

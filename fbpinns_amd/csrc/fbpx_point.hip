@@ -1,0 +1,514 @@
+// Point-level kernels: the inside test / take fill, the sorted POU reduction
+// and its transpose, and the masked Adam update.
+#include "fbpx_shapes.hpp"
+
+namespace fbpx {
+
+// ------------------------------------------------------------------ takes
+//
+// One thread per point; the boxes are staged through LDS in tiles.  The test
+// is the reference's inclusive FP32 comparison xmin <= x <= xmax on every
+// axis (decompositions.py:225-226); fill writes ascending box ids, which is
+// the row-major order of jnp.nonzero on the (n, m) inside matrix.
+
+constexpr int kBoxTile = 512;
+
+template <int XD, bool FILL>
+__global__ void __launch_bounds__(256) inside_kernel(const float* __restrict__ x, int n, const float* __restrict__ xmin,
+                                                     const float* __restrict__ xmax, int m,
+                                                     const uint8_t* __restrict__ mask, int32_t* __restrict__ counts,
+                                                     const int64_t* __restrict__ offsets, int32_t* __restrict__ n_take,
+                                                     int32_t* __restrict__ m_take) {
+  __shared__ float slo[kBoxTile * XD], shi[kBoxTile * XD];
+  __shared__ uint8_t smask[kBoxTile];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < n;
+  float xv[XD];
+#pragma unroll
+  for (int i = 0; i < XD; ++i) xv[i] = live ? x[(size_t)p * XD + i] : 0.f;
+  int cnt = 0;
+  int64_t pos = (FILL && live) ? offsets[p] : 0;
+  for (int t0 = 0; t0 < m; t0 += kBoxTile) {
+    const int tn = min(kBoxTile, m - t0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < tn * XD; i += blockDim.x) {
+      slo[i] = xmin[(size_t)t0 * XD + i];
+      shi[i] = xmax[(size_t)t0 * XD + i];
+    }
+    for (int i = threadIdx.x; i < tn; i += blockDim.x) smask[i] = mask ? mask[t0 + i] : 1;
+    __syncthreads();
+    if (live) {
+      for (int j = 0; j < tn; ++j) {
+        bool in = smask[j] != 0;
+#pragma unroll
+        for (int i = 0; i < XD; ++i) in = in && (xv[i] >= slo[j * XD + i]) && (xv[i] <= shi[j * XD + i]);
+        if (in) {
+          if (FILL) {
+            n_take[pos] = p;
+            m_take[pos] = t0 + j;
+            ++pos;
+          } else {
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (!FILL && live) counts[p] = cnt;
+}
+
+template <bool FILL>
+static int launch_inside(hipStream_t s, const float* x, int n, int xd, const float* xmin, const float* xmax, int m,
+                         const uint8_t* mask, int32_t* counts, const int64_t* offsets, int32_t* n_take,
+                         int32_t* m_take) {
+  if (n < 0 || m < 0 || xd < 1 || xd > FBPX_MAX_XD) {
+    set_error("inside: bad sizes n=%d m=%d xd=%d", n, m, xd);
+    return FBPX_EINVAL;
+  }
+  if (n == 0) return FBPX_OK;
+  if (!x || (m > 0 && (!xmin || !xmax)) || (!FILL && !counts) || (FILL && (!offsets || !n_take || !m_take))) {
+    set_error("inside: null pointer");
+    return FBPX_EINVAL;
+  }
+  dim3 grid((n + 255) / 256), block(256);
+  switch (xd) {
+    case 1: hipLaunchKernelGGL((inside_kernel<1, FILL>), grid, block, 0, s, x, n, xmin, xmax, m, mask, counts, offsets, n_take, m_take); break;
+    case 2: hipLaunchKernelGGL((inside_kernel<2, FILL>), grid, block, 0, s, x, n, xmin, xmax, m, mask, counts, offsets, n_take, m_take); break;
+    default: hipLaunchKernelGGL((inside_kernel<3, FILL>), grid, block, 0, s, x, n, xmin, xmax, m, mask, counts, offsets, n_take, m_take); break;
+  }
+  return hip_check(hipGetLastError(), FILL ? "inside_fill" : "inside_count");
+}
+
+// ------------------------------------------------------------------ POU
+
+template <class S>
+__global__ void __launch_bounds__(256) window_pairs_kernel(const float* __restrict__ dom, const float* __restrict__ x,
+                                                           fbpx_worklist wl, float* __restrict__ wpair) {
+  constexpr int J = S::J, XD = S::XD;
+  const int chunk = blockIdx.x;
+  const int model = wl.chunk_model[chunk];
+  const int beg = wl.chunk_start[chunk], end = wl.chunk_start[chunk + 1];
+  Dom d;
+  load_dom(dom, model, d);
+  for (int k = beg + threadIdx.x; k < end; k += blockDim.x) {
+    const int p = wl.pair_point[k];
+    float xv[XD], wj[J];
+#pragma unroll
+    for (int i = 0; i < XD; ++i) xv[i] = x[(size_t)p * XD + i];
+    window_jets<S>(xv, d, wj);
+#pragma unroll
+    for (int c = 0; c < J; ++c) wpair[(size_t)k * J + c] = wj[c];
+  }
+}
+
+template <int J>
+__global__ void __launch_bounds__(256) seg_sum_kernel(fbpx_worklist wl, const float* __restrict__ vals,
+                                                      float* __restrict__ out) {
+  const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (seg >= wl.n_segs) return;
+  float acc[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) acc[c] = 0.f;
+  for (int j = wl.seg_pair_start[seg]; j < wl.seg_pair_start[seg + 1]; ++j) {
+    const size_t id = wl.seg_pairs[j];
+#pragma unroll
+    for (int c = 0; c < J; ++c) acc[c] += vals[id * J + c];
+  }
+#pragma unroll
+  for (int c = 0; c < J; ++c) out[(size_t)seg * J + c] = acc[c];
+}
+
+// q = U / W on jets (trainers.py:165-166 under jvp):
+//   q0 = U0/W0, qi = (Ui - q0 Wi)/W0, qii = (Uii - 2 qi Wi - q0 Wii)/W0
+template <class S>
+__device__ inline void jet_div(const float* U, const float* W, float* q) {
+  const float iw = 1.f / W[0];
+  q[0] = U[0] * iw;
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (S::has1(i)) {
+      const int a = S::col1(i);
+      q[a] = (U[a] - q[0] * W[a]) * iw;
+      if (S::has2(i)) {
+        const int b = S::col2(i);
+        q[b] = (U[b] - 2.f * q[a] * W[a] - q[0] * W[b]) * iw;
+      }
+    }
+  }
+}
+
+// Transpose of jet_div w.r.t. U (W is parameter-independent).
+template <class S>
+__device__ inline void jet_div_T(const float* gq, const float* W, float* gU) {
+  const float iw = 1.f / W[0];
+  float g0 = gq[0];
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (S::has1(i)) {
+      const int a = S::col1(i);
+      float ga = gq[a];
+      if (S::has2(i)) {
+        const int b = S::col2(i);
+        const float gb = gq[b] * iw;
+        gU[b] = gb;
+        ga -= 2.f * W[a] * gb;
+        g0 -= W[b] * gb;
+      }
+      const float gaa = ga * iw;
+      gU[a] = gaa;
+      g0 -= W[a] * gaa;
+    }
+  }
+  gU[0] = g0 * iw;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
+                                                         const float* __restrict__ wsum, float* __restrict__ ujet) {
+  constexpr int J = S::J;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wl.n_points) return;
+  float acc[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) acc[c] = 0.f;
+  for (int seg = wl.point_seg_start[p]; seg < wl.point_seg_start[p + 1]; ++seg) {
+    float U[J], W[J], q[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) {
+      U[c] = 0.f;
+      W[c] = wsum[(size_t)seg * J + c];
+    }
+    for (int j = wl.seg_pair_start[seg]; j < wl.seg_pair_start[seg + 1]; ++j) {
+      const size_t id = wl.seg_pairs[j];
+#pragma unroll
+      for (int c = 0; c < J; ++c) U[c] += pairbuf[id * J + c];
+    }
+    jet_div<S>(U, W, q);
+#pragma unroll
+    for (int c = 0; c < J; ++c) acc[c] += q[c];
+  }
+  const float inv = 1.f / (float)wl.npou;
+#pragma unroll
+  for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = (wl.npou == 1) ? acc[c] : acc[c] * inv;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
+                                                           const float* __restrict__ g_ujet,
+                                                           float* __restrict__ gseg) {
+  constexpr int J = S::J;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wl.n_points) return;
+  const float inv = 1.f / (float)wl.npou;
+  float gq[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) {
+    const float v = g_ujet[(size_t)c * wl.n_points + p];
+    gq[c] = (wl.npou == 1) ? v : v * inv;
+  }
+  for (int seg = wl.point_seg_start[p]; seg < wl.point_seg_start[p + 1]; ++seg) {
+    float W[J], gU[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) W[c] = wsum[(size_t)seg * J + c];
+    jet_div_T<S>(gq, W, gU);
+#pragma unroll
+    for (int c = 0; c < J; ++c) gseg[(size_t)seg * J + c] = gU[c];
+  }
+}
+
+bool schema_supported(int xd, int d1, int d2) {
+#define X(XD, D1, D2) \
+  if (xd == XD && d1 == D1 && d2 == D2) return true;
+  FBPX_SCHEMAS(X)
+#undef X
+  return false;
+}
+
+// ------------------------------------------------------------------ Adam
+//
+// optax.adam (scale_by_adam, eps_root = 0, then scale_by_learning_rate and
+// apply_updates), element-wise in FP32 with the same operation order:
+//   mu = omb1*g + b1*mu ; nu = omb2*g^2 + b2*nu ; c = count + 1
+//   p  = p + nlr * ((mu / (1 - b1^c)) / (sqrt(nu / (1 - b2^c)) + eps))
+// omb1 = f32(1 - b1) and nlr = f32(-lr) are formed in double on the host, as
+// Python forms them before they meet the FP32 arrays.
+
+struct AdamConst {
+  float b1, b2, omb1, omb2, nlr, eps;
+};
+
+__device__ inline void adam_elem(float& p, float& m, float& v, float g, float bc1, float bc2, const AdamConst& k) {
+#pragma clang fp contract(off)
+  m = k.omb1 * g + k.b1 * m;
+  v = k.omb2 * (g * g) + k.b2 * v;
+  const float mh = m / bc1;
+  const float vh = v / bc2;
+  const float u = mh / (sqrtf(vh) + k.eps);
+  p = p + k.nlr * u;
+}
+
+__global__ void __launch_bounds__(256) reduce_rows_kernel(int P, const float* __restrict__ partial,
+                                                          const int32_t* __restrict__ rows,
+                                                          const int32_t* __restrict__ rcs, int n_rows,
+                                                          float* __restrict__ grad) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_rows * P) return;
+  const int row = (int)(idx / P), e = (int)(idx % P);
+  float g = 0.f;
+  for (int c = rcs[row]; c < rcs[row + 1]; ++c) g += partial[(size_t)c * P + e];
+  grad[(size_t)rows[row] * P + e] = g;
+}
+
+__global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ theta, float* __restrict__ mu,
+                                                        float* __restrict__ nu, int P,
+                                                        const float* __restrict__ grad,
+                                                        const int32_t* __restrict__ rows, int n_rows,
+                                                        const int32_t* __restrict__ count, AdamConst k) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_rows * P) return;
+  const int row = (int)(idx / P), e = (int)(idx % P);
+  const float cf = (float)(count[0] + 1);
+  const float bc1 = 1.f - powf(k.b1, cf);
+  const float bc2 = 1.f - powf(k.b2, cf);
+  const size_t i = (size_t)rows[row] * P + e;
+  float p = theta[i], m = mu[i], v = nu[i];
+  adam_elem(p, m, v, grad[i], bc1, bc2, k);
+  theta[i] = p;
+  mu[i] = m;
+  nu[i] = v;
+}
+
+__global__ void __launch_bounds__(256) adam_flat_kernel(float* __restrict__ p, float* __restrict__ mu,
+                                                        float* __restrict__ nu, const float* __restrict__ g, int n,
+                                                        const int32_t* __restrict__ count, AdamConst k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float cf = (float)(count[0] + 1);
+  const float bc1 = 1.f - powf(k.b1, cf);
+  const float bc2 = 1.f - powf(k.b2, cf);
+  float pv = p[i], m = mu[i], v = nu[i];
+  adam_elem(pv, m, v, g[i], bc1, bc2, k);
+  p[i] = pv;
+  mu[i] = m;
+  nu[i] = v;
+}
+
+// Deterministic mean: stage 1 sums a fixed grid of strided ranges into
+// scratch[kSumBlocks] with an LDS tree, stage 2 reduces those in one block.
+// No atomics and no memset, so it is safe inside a captured graph.
+constexpr int kSumBlocks = 256;
+
+__device__ inline float block_tree_sum(float s, float* sm) {
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  return sm[0];
+}
+
+__global__ void __launch_bounds__(256) sum_stage1_kernel(const float* __restrict__ x, int64_t n,
+                                                         float* __restrict__ scratch) {
+  __shared__ float sm[256];
+  float s = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i];
+  const float t = block_tree_sum(s, sm);
+  if (threadIdx.x == 0) scratch[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict__ scratch, int nb, int64_t n,
+                                                         float* __restrict__ out) {
+  __shared__ float sm[256];
+  const float s = (int)threadIdx.x < nb ? scratch[threadIdx.x] : 0.f;
+  const float t = block_tree_sum(s, sm);
+  if (threadIdx.x == 0) out[0] = t / (float)n;
+}
+
+__global__ void count_increment_kernel(int32_t* count) {
+  if (threadIdx.x == 0) count[0] = count[0] + 1;
+}
+
+static AdamConst adam_const(double lr, double b1, double b2, double eps) {
+  AdamConst k;
+  k.b1 = (float)b1;
+  k.b2 = (float)b2;
+  k.omb1 = (float)(1.0 - b1);
+  k.omb2 = (float)(1.0 - b2);
+  k.nlr = (float)(-lr);
+  k.eps = (float)eps;
+  return k;
+}
+
+}  // namespace fbpx
+
+using namespace fbpx;
+
+static int check_net_schema(const fbpx_net* net, int* J) {
+  if (!net) {
+    set_error("null net");
+    return FBPX_EINVAL;
+  }
+  if (!schema_supported(net->xd, net->d1mask, net->d2mask)) {
+    set_error("jet schema xd=%d d1mask=%d d2mask=%d not compiled", net->xd, net->d1mask, net->d2mask);
+    return FBPX_EUNSUPPORTED;
+  }
+  *J = 1 + popc(net->d1mask) + popc(net->d2mask);
+  return FBPX_OK;
+}
+
+static int check_wl(const fbpx_worklist* wl) {
+  if (!wl || wl->n_pairs < 0 || wl->n_chunks < 0 || wl->n_points < 0 || wl->n_segs < 0 || wl->npou < 1) {
+    set_error("bad work list");
+    return FBPX_EINVAL;
+  }
+  return FBPX_OK;
+}
+
+extern "C" {
+
+int fbpx_inside_count(void* stream, const float* x, int32_t n, int32_t xd, const float* xmin, const float* xmax,
+                      int32_t m, const uint8_t* model_mask, int32_t* counts) {
+  return launch_inside<false>((hipStream_t)stream, x, n, xd, xmin, xmax, m, model_mask, counts, nullptr, nullptr,
+                              nullptr);
+}
+
+int fbpx_inside_fill(void* stream, const float* x, int32_t n, int32_t xd, const float* xmin, const float* xmax,
+                     int32_t m, const uint8_t* model_mask, const int64_t* offsets, int32_t* n_take,
+                     int32_t* m_take) {
+  return launch_inside<true>((hipStream_t)stream, x, n, xd, xmin, xmax, m, model_mask, nullptr, offsets, n_take,
+                             m_take);
+}
+
+int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom, const float* x, const fbpx_worklist* wl,
+                     float* pair_scratch, float* wsum) {
+  int J, rc;
+  if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (wl->n_chunks > 0) {
+#define X(XD, D1, D2)                                                                                        \
+  if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                               \
+    hipLaunchKernelGGL((window_pairs_kernel<Schema<XD, D1, D2>>), dim3(wl->n_chunks), dim3(256), 0, s, dom, x, \
+                       *wl, pair_scratch);
+    FBPX_SCHEMAS(X)
+#undef X
+  }
+  if ((rc = hip_check(hipGetLastError(), "window_pairs"))) return rc;
+  if (wl->n_segs > 0) {
+    dim3 grid((wl->n_segs + 255) / 256);
+    switch (J) {
+#define C(JJ) \
+  case JJ: hipLaunchKernelGGL((seg_sum_kernel<JJ>), grid, dim3(256), 0, s, *wl, pair_scratch, wsum); break;
+      C(1) C(2) C(3) C(4) C(5) C(6) C(7)
+#undef C
+      default: set_error("J=%d", J); return FBPX_EUNSUPPORTED;
+    }
+  }
+  return hip_check(hipGetLastError(), "seg_sum");
+}
+
+int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* pairbuf,
+                    const float* wsum, float* ujet) {
+  int J, rc;
+  if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  if (wl->n_points == 0) return FBPX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((wl->n_points + 255) / 256);
+#define X(XD, D1, D2)                                                                                      \
+  if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                             \
+    hipLaunchKernelGGL((pou_reduce_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, pairbuf, wsum, ujet);
+  FBPX_SCHEMAS(X)
+#undef X
+  return hip_check(hipGetLastError(), "pou_reduce");
+}
+
+int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* wsum,
+                      const float* g_ujet, float* gseg) {
+  int J, rc;
+  if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  if (wl->n_points == 0) return FBPX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((wl->n_points + 255) / 256);
+#define X(XD, D1, D2)                                                                                          \
+  if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                                 \
+    hipLaunchKernelGGL((pou_backward_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, wsum, g_ujet, gseg);
+  FBPX_SCHEMAS(X)
+#undef X
+  return hip_check(hipGetLastError(), "pou_backward");
+}
+
+int fbpx_reduce_rows(void* stream, int32_t P, const float* partial, const int32_t* rows,
+                     const int32_t* row_chunk_start, int32_t n_rows, float* grad) {
+  if (P <= 0 || n_rows < 0) {
+    set_error("reduce_rows: bad sizes P=%d n_rows=%d", P, n_rows);
+    return FBPX_EINVAL;
+  }
+  if (n_rows == 0) return FBPX_OK;
+  if (!partial || !rows || !row_chunk_start || !grad) {
+    set_error("reduce_rows: null pointer");
+    return FBPX_EINVAL;
+  }
+  const int64_t total = (int64_t)n_rows * P;
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     P, partial, rows, row_chunk_start, n_rows, grad);
+  return hip_check(hipGetLastError(), "reduce_rows");
+}
+
+int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
+                   const int32_t* rows, int32_t n_rows, const int32_t* count, double lr, double b1, double b2,
+                   double eps) {
+  if (P <= 0 || n_rows < 0) {
+    set_error("adam_rows: bad sizes P=%d n_rows=%d", P, n_rows);
+    return FBPX_EINVAL;
+  }
+  if (n_rows == 0) return FBPX_OK;
+  if (!theta || !mu || !nu || !grad || !rows || !count) {
+    set_error("adam_rows: null pointer");
+    return FBPX_EINVAL;
+  }
+  const int64_t total = (int64_t)n_rows * P;
+  hipLaunchKernelGGL(adam_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     theta, mu, nu, P, grad, rows, n_rows, count, adam_const(lr, b1, b2, eps));
+  return hip_check(hipGetLastError(), "adam_rows");
+}
+
+int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g, int32_t n, const int32_t* count,
+                   double lr, double b1, double b2, double eps) {
+  if (n < 0) {
+    set_error("adam_flat: n=%d", n);
+    return FBPX_EINVAL;
+  }
+  if (n == 0) return FBPX_OK;
+  if (!p || !mu || !nu || !g || !count) {
+    set_error("adam_flat: null pointer");
+    return FBPX_EINVAL;
+  }
+  hipLaunchKernelGGL(adam_flat_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, mu, nu, g, n,
+                     count, adam_const(lr, b1, b2, eps));
+  return hip_check(hipGetLastError(), "adam_flat");
+}
+
+int fbpx_count_increment(void* stream, int32_t* count) {
+  if (!count) {
+    set_error("count_increment: null pointer");
+    return FBPX_EINVAL;
+  }
+  hipLaunchKernelGGL(count_increment_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, count);
+  return hip_check(hipGetLastError(), "count_increment");
+}
+
+int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out) {
+  if (n <= 0 || !x || !scratch || !out) {
+    set_error("mean: bad arguments (n=%lld)", (long long)n);
+    return FBPX_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t need = (n + 255) / 256;
+  const int nb = (int)(need < kSumBlocks ? need : kSumBlocks);
+  hipLaunchKernelGGL(sum_stage1_kernel, dim3(nb), dim3(256), 0, s, x, n, scratch);
+  hipLaunchKernelGGL(sum_stage2_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n, out);
+  return hip_check(hipGetLastError(), "mean");
+}
+
+}  // extern "C"

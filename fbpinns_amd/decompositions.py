@@ -1,0 +1,165 @@
+"""Domain decompositions (reference: decompositions.py).
+
+RectangularDecompositionND (and its multilevel form) is what the HIP kernels
+accelerate: the per-pair norm_fn / unnorm_fn / window_fn below are the
+API-level definitions for ONE point and ONE subdomain (decompositions.py:
+183-199); the kernels implement the same functions for every pair at once,
+and the inside tests run on the GPU (fbpx_inside_count / fbpx_inside_fill).
+"""
+
+import numpy as np
+import torch
+
+from fbpinns_amd import networks, ops, windows
+
+
+class Decomposition:
+    """Base decomposition class (decompositions.py:31-94)."""
+
+    @staticmethod
+    def init_params(*args):
+        raise NotImplementedError
+
+    @staticmethod
+    def norm_fn(params, x):
+        raise NotImplementedError
+
+    @staticmethod
+    def unnorm_fn(params, u):
+        raise NotImplementedError
+
+    @staticmethod
+    def window_fn(params, x):
+        raise NotImplementedError
+
+    @staticmethod
+    def inside_points(all_params, x_batch):
+        """Returns n_take, m_take, inside_ims."""
+        raise NotImplementedError
+
+    @staticmethod
+    def inside_models(all_params, x_batch, ims):
+        """Returns inside_ips, d."""
+        raise NotImplementedError
+
+
+class RectangularDecompositionND(Decomposition):
+    """Hyperrectangular subdomains (decompositions.py:101-227)."""
+
+    @staticmethod
+    def init_params(subdomain_xs, subdomain_ws, unnorm):
+        xd = len(subdomain_xs)
+        ps = RectangularDecompositionND._get_level_params(0, xd, subdomain_xs, subdomain_ws, unnorm)
+        return RectangularDecompositionND._static(ps, xd)
+
+    @staticmethod
+    def _static(ps, xd):
+        m = ps[0].shape[0]
+        xmins0, xmaxs0 = ps[0] + ps[2] / 2, ps[1] - ps[3] / 2
+        params = [torch.tensor(p, dtype=torch.float32) for p in ps]
+        static = {
+            "m": int(m),
+            "xd": xd,
+            "subdomain": {"params": params[:-1], "pou": params[-1]},
+            "xmins0": xmins0,
+            "xmaxs0": xmaxs0,
+        }
+        return static, {}
+
+    @staticmethod
+    def _get_level_params(il, xd, subdomain_xs, subdomain_ws, unnorm):
+        """decompositions.py:135-181 (float64 numpy)."""
+        xs = np.stack(np.meshgrid(*[np.asarray(x, dtype=np.float64) for x in subdomain_xs], indexing="ij"), 0)
+        ws = np.stack(np.meshgrid(*[np.asarray(w, dtype=np.float64) for w in subdomain_ws], indexing="ij"), 0)
+        if xs.shape != ws.shape:
+            raise ValueError("shape of subdomain_ws not same as subdomain_xs")
+        xmins, xmaxs = xs - ws / 2, xs + ws / 2
+        wmins, wmaxs = 0.5 * (xmaxs - xmins), 0.5 * (xmaxs - xmins)
+        for i in range(xd):
+            lo, hi = [slice(None)] * xd, [slice(None)] * xd
+            lo[i], hi[i] = slice(None, -1), slice(1, None)
+            lo, hi = (i,) + tuple(lo), (i,) + tuple(hi)
+            wmaxs[lo] = wmins[hi] = xmaxs[lo] - xmins[hi]
+            e0, e1 = [slice(None)] * xd, [slice(None)] * xd
+            e0[i], e1[i] = 0, -1
+            e0, e1 = (i,) + tuple(e0), (i,) + tuple(e1)
+            wmins[e0] = wmaxs[e0]
+            wmaxs[e1] = wmins[e1]
+        if (wmins <= 0).any() or (wmaxs <= 0).any():
+            raise ValueError("some subdomains are not overlapping!")
+        flat = lambda v: v.reshape(xd, -1).T
+        xmins, xmaxs, wmins, wmaxs = flat(xmins), flat(xmaxs), flat(wmins), flat(wmaxs)
+        s = (xmins.shape[0], 1)
+        flags = np.zeros(s) if xmins.shape[0] == 1 else np.ones(s)
+        unnorms = np.concatenate([unnorm[0] * np.ones(s), unnorm[1] * np.ones(s)], axis=1)
+        pous = il * np.ones(s)
+        return [xmins, xmaxs, wmins, wmaxs, flags, unnorms, pous]
+
+    # per-(point, subdomain) functions, for ONE point and ONE subdomain
+    @staticmethod
+    def norm_fn(params, x):
+        xmin, xmax = params["static"]["decomposition"]["subdomain"]["params"][:2]
+        return networks.norm((xmax + xmin) / 2, (xmax - xmin) / 2, x)
+
+    @staticmethod
+    def unnorm_fn(params, u):
+        mu, sd = params["static"]["decomposition"]["subdomain"]["params"][5]
+        return networks.unnorm(mu, sd, u)
+
+    @staticmethod
+    def window_fn(params, x):
+        p = params["static"]["decomposition"]["subdomain"]["params"]
+        return p[4] * windows.cosine(p[0], p[1], x) + (1 - p[4])
+
+    # inside tests (GPU kernels)
+    @staticmethod
+    def _boxes(all_params, device):
+        p = all_params["static"]["decomposition"]["subdomain"]["params"]
+        return p[0].to(device).contiguous(), p[1].to(device).contiguous()
+
+    @staticmethod
+    def inside_points(all_params, x_batch):
+        """decompositions.py:201-208: all (point, subdomain) memberships in
+        row-major nonzero order.  x_batch: (n, xd) float32 GPU tensor.
+        Returns n_take, m_take (int64 numpy) and inside_ims (numpy)."""
+        xmins, xmaxs = RectangularDecompositionND._boxes(all_params, x_batch.device)
+        m = xmins.shape[0]
+        n_take, m_take, _ = ops.inside_takes(x_batch.contiguous(), xmins, xmaxs)
+        n_take = n_take.cpu().numpy().astype(np.int64)
+        m_take = m_take.cpu().numpy().astype(np.int64)
+        inside_ims = np.nonzero(np.bincount(m_take, minlength=m))[0]
+        return n_take, m_take, inside_ims
+
+    @staticmethod
+    def inside_models(all_params, x_batch, ims):
+        """decompositions.py:210-215: points inside >= 1 model of ims and the
+        average number of points per model per dimension."""
+        xmins, xmaxs = RectangularDecompositionND._boxes(all_params, x_batch.device)
+        m, xd = xmins.shape
+        ims = np.asarray(ims, dtype=np.int64)
+        mask = np.zeros(m, dtype=np.uint8)
+        mask[ims] = 1
+        maskd = torch.tensor(mask, device=x_batch.device)
+        n_take, m_take, counts = ops.inside_takes(x_batch.contiguous(), xmins, xmaxs, maskd)
+        inside_ips = np.nonzero(counts.cpu().numpy() > 0)[0]
+        per_model = np.bincount(m_take.cpu().numpy().astype(np.int64), minlength=m)[ims].astype(np.float64)
+        d = per_model.mean() ** (1 / xd) if len(ims) else float("nan")
+        return inside_ips, d
+
+
+class MultilevelRectangularDecompositionND(RectangularDecompositionND):
+    """Several rectangular levels, level il has pou index il
+    (decompositions.py:338-375)."""
+
+    @staticmethod
+    def init_params(subdomain_xss, subdomain_wss, unnorm):
+        nms = [tuple(len(x) for x in xs) for xs in subdomain_xss]
+        if False in [len(nm) == len(nms[0]) for nm in nms]:
+            raise ValueError("subdomain_xss are not all the same dimensionality")
+        xd = len(subdomain_xss[0])
+        ps = [[] for _ in range(7)]
+        for il, (xs, ws) in enumerate(zip(subdomain_xss, subdomain_wss)):
+            for i, p_ in enumerate(RectangularDecompositionND._get_level_params(il, xd, xs, ws, unnorm)):
+                ps[i].append(p_)
+        ps = [np.concatenate(p) for p in ps]
+        return RectangularDecompositionND._static(ps, xd)

@@ -1,0 +1,314 @@
+"""The fused FBPINN train step (replaces FBPINN_update, trainers.py:285-296).
+
+State lives on one GPU in packed form:
+  theta [m][P]  every subdomain's FCN parameters (layout in include/fbpx.h);
+                all_params["trainable"]["network"]["subdomain"]["layers"]
+                holds (W, b) views into it, in the reference's shapes
+                (W (m, out, in), b (m, out)).
+  mu, nu [m][P] Adam moments; count int32[1] (shared, as optax's).
+  pflat [Q]     the problem's trainable leaves (e.g. mu of the inverse
+                problem), with their Adam moments.
+  dom [m][12]   xmin, xmax, unnorm mean/sd and window flag per subdomain.
+
+One step = fbpx_forward -> fbpx_pou_reduce -> loss epilogue (torch on the
+per-point jets) -> autograd (fbpx_pou_backward -> fbpx_backward ->
+fbpx_reduce_rows, plus torch for the epilogue) -> fbpx_adam_rows over the
+trained rows (+ fbpx_adam_flat) -> fbpx_count_increment.  With graph=True
+the whole step is captured once per work list into a hipGraph and replayed.
+"""
+
+import numpy as np
+import torch
+
+from fbpinns_amd import ops
+from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
+from fbpinns_amd.epilogue import Epilogue, schema_masks
+from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
+from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch
+from fbpinns_amd.util.logger import logger
+
+# compiled jet schemas per input dimension (fbpx_shapes.hpp)
+SCHEMAS = {
+    1: [(0, 0), (1, 0), (1, 1)],
+    2: [(0, 0), (3, 0), (3, 1), (3, 2), (3, 3)],
+    3: [(0, 0), (7, 0), (7, 3), (7, 7)],
+}
+
+
+def pick_schema(xd, d1, d2):
+    """Smallest compiled schema carrying every requested derivative."""
+    best = None
+    for a, b in SCHEMAS[xd]:
+        if (a & d1) == d1 and (b & d2) == d2:
+            j = 1 + bin(a).count("1") + bin(b).count("1")
+            if best is None or j < best[0]:
+                best = (j, a, b)
+    if best is None:
+        raise FbpxError(f"no compiled jet schema carries d1mask={d1} d2mask={d2} for xd={xd}")
+    return best[1], best[2]
+
+
+def tree_to(t, device):
+    if isinstance(t, dict):
+        return {k: tree_to(v, device) for k, v in t.items()}
+    if isinstance(t, list):
+        return [tree_to(v, device) for v in t]
+    if isinstance(t, tuple):
+        return tuple(tree_to(v, device) for v in t)
+    if isinstance(t, torch.Tensor):
+        return t.to(device)
+    return t
+
+
+def network_shape(layers):
+    """(xd, h, nh) of a stacked FCN, or raise if it is not [xd, h x nh, 1]."""
+    sizes = [layers[0][0].shape[2]] + [W.shape[1] for W, _ in layers]
+    xd, hs, ud = sizes[0], sizes[1:-1], sizes[-1]
+    if ud != 1:
+        raise FbpxError(f"the fused engine computes scalar solutions; network output size is {ud}")
+    if len(set(hs)) != 1 or len(hs) not in (1, 2):
+        raise FbpxError(f"layer sizes {sizes} not compiled: need [xd, h] or [xd, h, h] hidden layers")
+    if xd < 1 or xd > FBPX_MAX_XD:
+        raise FbpxError(f"input dimension {xd} not compiled (1..{FBPX_MAX_XD})")
+    return xd, hs[0], len(hs)
+
+
+class FusedEngine:
+    """Owns the packed device state and runs train steps for one problem."""
+
+    def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
+                 graph=True):
+        from fbpinns_amd.decompositions import RectangularDecompositionND
+        from fbpinns_amd.networks import FCN
+        if not torch.cuda.is_available():
+            raise FbpxError("the fused engine needs a GPU (torch.cuda.is_available() is False)")
+        if not (isinstance(decomposition, type) and issubclass(decomposition, RectangularDecompositionND)):
+            raise FbpxError("the fused engine accelerates RectangularDecompositionND (and its multilevel form)")
+        if not (isinstance(network, type) and issubclass(network, FCN)):
+            raise FbpxError("the fused engine accelerates the tanh FCN network")
+        self.problem, self.decomposition, self.network = problem, decomposition, network
+        self.device = torch.device(device)
+        self.graph_enabled = bool(graph)
+        ok = dict(optimiser_kwargs or {})
+        self.lr = float(ok.pop("learning_rate", 1e-3))
+        self.b1 = float(ok.pop("b1", 0.9))
+        self.b2 = float(ok.pop("b2", 0.999))
+        self.eps = float(ok.pop("eps", 1e-8))
+        if float(ok.pop("eps_root", 0.0)) != 0.0 or ok.pop("nesterov", False) or ok:
+            raise FbpxError(f"unsupported Adam options {ok}")
+
+        ap = tree_to(all_params, self.device)
+        self.all_params = ap
+        dec = ap["static"]["decomposition"]
+        self.m, self.xd = int(dec["m"]), int(dec["xd"])
+        layers = ap["trainable"]["network"]["subdomain"]["layers"]
+        xd, h, nh = network_shape(layers)
+        if xd != self.xd:
+            raise FbpxError("network input size != decomposition dimension")
+        self.net0 = NetSpec(xd, h, nh)
+        self.P = self.net0.P
+        self.theta = torch.empty((self.m, self.P), dtype=torch.float32, device=self.device)
+        views = []
+        off = 0
+        for W, b in layers:
+            o, i = W.shape[1], W.shape[2]
+            Wv = self.theta[:, off:off + o * i].view(self.m, o, i)
+            Wv.copy_(W)
+            off += o * i
+            bv = self.theta[:, off:off + o]
+            bv.copy_(b)
+            off += o
+            views.append((Wv, bv))
+        assert off == self.P
+        ap["trainable"]["network"]["subdomain"]["layers"] = views
+        self.mu = torch.zeros_like(self.theta)
+        self.nu = torch.zeros_like(self.theta)
+        self.count = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+        # problem trainable leaves -> one flat vector with views
+        prob = ap["trainable"].get("problem", {})
+        self.pkeys = list(prob.keys())
+        self.pshapes = [tuple(torch.as_tensor(prob[k]).shape) for k in self.pkeys]
+        Q = sum(int(np.prod(s)) for s in self.pshapes)
+        self.pflat = torch.zeros(max(Q, 1), dtype=torch.float32, device=self.device)
+        self.Q = Q
+        off = 0
+        for k, s in zip(self.pkeys, self.pshapes):
+            n = int(np.prod(s))
+            self.pflat[off:off + n].copy_(torch.as_tensor(prob[k], dtype=torch.float32).reshape(-1))
+            prob[k] = self.pflat[off:off + n].view(s)
+            off += n
+        self.pmu = torch.zeros_like(self.pflat)
+        self.pnu = torch.zeros_like(self.pflat)
+
+        # per-subdomain static rows
+        p = dec["subdomain"]["params"]
+        dom = torch.zeros((self.m, FBPX_DOM_STRIDE), dtype=torch.float32, device=self.device)
+        dom[:, 0:self.xd] = p[0]
+        dom[:, 4:4 + self.xd] = p[1]
+        dom[:, 8] = p[5][:, 0]
+        dom[:, 9] = p[5][:, 1]
+        dom[:, 10] = p[4][:, 0]
+        self.dom = dom.contiguous()
+        self.plan = None
+        self._graph = None
+        self._test_cache = {}
+
+    # -------------------------------------------------------------- params
+
+    def _params_for_step(self):
+        """all_params whose trainable problem leaves are differentiable views
+        of a leaf that shares pflat's storage."""
+        ap = self.all_params
+        if not self.Q:
+            return ap, None
+        leaf = self.pflat.detach().requires_grad_(True)
+        prob = {}
+        off = 0
+        for k, s in zip(self.pkeys, self.pshapes):
+            n = int(np.prod(s))
+            prob[k] = leaf[off:off + n].view(s)
+            off += n
+        tr = dict(ap["trainable"])
+        tr["problem"] = prob
+        return {"static": ap["static"], "trainable": tr}, leaf
+
+    # -------------------------------------------------------------- set-up
+
+    def set_constraints(self, constraints_global, required_ujss):
+        """constraints_global: [[x_c, *values_c], ...] (CPU or GPU tensors)."""
+        self.cglobal = [[tree_to(c_, self.device) for c_ in c] for c in constraints_global]
+        self.required_ujss = [tuple((iu, tuple(ixs)) for iu, ixs in r) for r in required_ujss]
+        self.x_global = torch.cat([c[0] for c in self.cglobal]).to(torch.float32).contiguous()
+        sizes = [c[0].shape[0] for c in self.cglobal]
+        self.c_sizes = sizes
+        self.c_offsets = list(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int))
+        d1, d2 = schema_masks(self.required_ujss, self.xd)
+        d1, d2 = pick_schema(self.xd, d1, d2)
+        self.net = self.net0.with_schema(d1, d2)
+        if not self.net.supported():
+            raise FbpxError(f"no kernels for {self.net}")
+
+    def set_active(self, active):
+        """New scheduler state: rebuild takes, work list, epilogue and graph
+        (trainers.py:509-575 and the recompile at :647-653)."""
+        active = np.asarray(active).astype(np.int64)
+        training_ips, cfs, cips, dstat = get_x_batch(active, self.all_params, self.x_global, self.c_offsets,
+                                                     self.c_sizes, self.decomposition)
+        tips = torch.as_tensor(training_ips, dtype=torch.long, device=self.device)
+        x = self.x_global[tips].contiguous()
+        cranges, cvalues = [], []
+        off = 0
+        for ic, ips in enumerate(cips):
+            n_c = len(ips)
+            loc = torch.as_tensor(training_ips[ips] - self.c_offsets[ic], dtype=torch.long, device=self.device)
+            cvalues.append([x[off:off + n_c]] + [v[loc] for v in self.cglobal[ic][1:]])
+            cranges.append((off, n_c))
+            off += n_c
+        takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
+        n_active = int((active_ == 1).sum())
+        plan = build_worklist(takes, all_ims, n_active, x.shape[0], self.device)
+        wsum = torch.empty((max(plan.wl.n_segs, 1), self.net.J), dtype=torch.float32, device=self.device)
+        scratch = torch.empty((max(plan.wl.n_pairs, 1), self.net.J), dtype=torch.float32, device=self.device)
+        ops.pou_prepare(self.net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)
+        del scratch
+        self.plan = ops.Plan(self.net, self.dom, x, plan.wl, wsum, plan.chunk_train, plan.rows,
+                             plan.row_chunk_start, self.m)
+        self.workplan = plan
+        self.active = active_
+        self.x = x
+        self.takes = takes
+        self.epilogue = Epilogue(self.problem, self.all_params, x, cranges, cvalues, self.required_ujss, self.net)
+        self.n_active = n_active
+        logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks, "
+                    f"{n_active} trained / {len(all_ims)} used subdomains, jets J={self.net.J}, "
+                    f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}, d={dstat:.2f}")
+        self._graph = None
+        if self.graph_enabled:
+            self._capture()
+        return dstat
+
+    # -------------------------------------------------------------- step
+
+    def _step_body(self):
+        plan = self.plan
+        th = self.theta.detach().requires_grad_(True)
+        ujet = FBPINNJets.apply(th, plan)
+        params, pleaf = self._params_for_step()
+        loss = self.epilogue.loss(ujet, params)
+        if pleaf is not None:
+            g_theta, g_p = torch.autograd.grad(loss, [th, pleaf], allow_unused=True)
+        else:
+            (g_theta,) = torch.autograd.grad(loss, [th])
+            g_p = None
+        ops.adam_rows(self.theta, self.mu, self.nu, g_theta, plan.rows, self.count, self.lr, self.b1, self.b2,
+                      self.eps)
+        if self.Q:
+            if g_p is None:
+                g_p = torch.zeros_like(self.pflat)
+            ops.adam_flat(self.pflat, self.pmu, self.pnu, g_p.contiguous(), self.count, self.lr, self.b1, self.b2,
+                          self.eps)
+        ops.count_increment(self.count)
+        return loss.detach()
+
+    def _state(self):
+        return [self.theta, self.mu, self.nu, self.pflat, self.pmu, self.pnu, self.count]
+
+    def _capture(self):
+        saved = [t.clone() for t in self._state()]
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(2):
+                self._step_body()
+        torch.cuda.current_stream().wait_stream(s)
+        for t, v in zip(self._state(), saved):
+            t.copy_(v)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._graph_loss = self._step_body()
+        self._graph = g
+        torch.cuda.synchronize()
+
+    def step(self):
+        """One train step; returns the loss as a device scalar."""
+        if self.plan is None:
+            raise FbpxError("set_active() must be called before step()")
+        if self._graph is not None:
+            self._graph.replay()
+            return self._graph_loss
+        return self._step_body()
+
+    # -------------------------------------------------------------- inference
+
+    def make_plan(self, x, active=None, schema=(0, 0)):
+        """Plan (work list + buffers) for points x with every subdomain (or
+        `active`) used and every used subdomain trained."""
+        x = x.to(self.device, torch.float32).contiguous()
+        active = np.ones(self.m, dtype=int) if active is None else np.asarray(active)
+        takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
+        net = self.net0.with_schema(*schema)
+        plan = build_worklist(takes, all_ims, int((active_ == 1).sum()), x.shape[0], self.device)
+        wsum = torch.empty((max(plan.wl.n_segs, 1), net.J), dtype=torch.float32, device=self.device)
+        scratch = torch.empty((max(plan.wl.n_pairs, 1), net.J), dtype=torch.float32, device=self.device)
+        ops.pou_prepare(net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)
+        return ops.Plan(net, self.dom, x, plan.wl, wsum, plan.chunk_train, plan.rows, plan.row_chunk_start,
+                        self.m), takes
+
+    def model_jets(self, x, active=None, schema=(0, 0)):
+        """Jets of u~ (before constraining_fn) at points x; returns (J, n)."""
+        x = x.to(self.device, torch.float32).contiguous()
+        key = (x.data_ptr(), x.shape[0], None if active is None else tuple(np.asarray(active).tolist()), schema)
+        if key not in self._test_cache:
+            self._test_cache = {key: (x, self.make_plan(x, active, schema)[0])}
+        x, plan = self._test_cache[key]
+        with torch.no_grad():
+            return FBPINNJets.apply(self.theta, plan)
+
+    def model(self, x, active=None):
+        """Constrained FBPINN solution u (n, 1) at x (FBPINN_model, trainers.py:126-177)."""
+        x = x.to(self.device, torch.float32).contiguous()
+        ujet = self.model_jets(x, active)
+        params, _ = self._params_for_step()
+        with torch.no_grad():
+            return self.problem.constraining_fn(params, x, ujet[0][:, None])

@@ -1,0 +1,251 @@
+"""Loss epilogue: from the kernels' per-point jets of the POU-combined network
+output (u~ and its derivatives) to the problem's loss.
+
+The reference applies `constraining_fn` inside the model and differentiates
+the whole model with one nested forward-mode jvp per leaf of the derivative
+tree (trainers.py:63-107, 174, 197-247).  Here the expensive part (the
+subdomain networks and the POU sum) arrives as exact jets from the HIP
+kernels, and only the cheap constraining function is differentiated:
+
+* affine path (constraining_fn(x, u) = A(x) + B(x) u with A, B independent of
+  trainable parameters, detected numerically at set-up): the jets of A and B
+  are computed once, in float64, with the same nested jvp, and each step
+  applies the Leibniz rule;
+* generic path (anything else): u~ is replaced near each point by its exact
+  second-order Taylor polynomial in the requested axes, and the reference's
+  nested jvp tree is evaluated on constraining_fn(x, poly(x)), which gives
+  the same derivatives for every request of order <= 2 with repeated indices.
+
+Then loss_fn(all_params, constraints_) is evaluated exactly as the reference
+calls it (trainers.py:258-267).
+"""
+
+import torch
+from torch.func import jvp
+
+from fbpinns_amd.ops import FbpxError
+
+
+# ------------------------------------------------------------------ derivative tree
+
+def get_jmaps(required_ujs):
+    """Derivative tree of required_ujs (trainers.py:63-107)."""
+    tree = {}
+    for iu, ixs in required_ujs:
+        t = tree
+        for ix in ixs:
+            t = t.setdefault(ix, {})
+
+    def nodes_of(t, n=(), ks=()):
+        ni = len(n)
+        for k in t:
+            ks_ = ks + (k,)
+            if t[k]:
+                n += (((ni, k), ks_, 0),)
+                n = nodes_of(t[k], n, ks_)
+            else:
+                n += (((ni, k), ks_, 1),)
+        return n
+
+    nodes = nodes_of(tree)
+    leaves = tuple((i + 1, node[1]) for i, node in enumerate(nodes) if node[2]) or ((0, ()),)
+    jac_is = tuple(([leaf[1][:len(ixs)] for leaf in leaves].index(ixs), len(ixs), iu) for iu, ixs in required_ujs)
+    return nodes, leaves, jac_is
+
+
+def get_ujs(x_batch, jmaps, u):
+    """Evaluate the nested-jvp tree (trainers.py:213-247); u(x) -> ((n, ud), ())."""
+    nodes, leaves, jac_is = jmaps
+    n, xd = x_batch.shape
+    eye = torch.eye(xd, dtype=x_batch.dtype, device=x_batch.device)
+
+    def jacfwd(f, v):
+        def jacfun(x):
+            y, j, aux = jvp(f, (x,), (v,), has_aux=True)
+            return j, aux + (y,)
+        return jacfun
+
+    fs = [u]
+    for (ni, ix), _, _ in nodes:
+        fs.append(jacfwd(fs[ni], eye[ix].expand(n, xd)))
+    jacs = []
+    for ie, _ in leaves:
+        fin, jac = fs[ie](x_batch)
+        jacs.append(jac + (fin,))
+    return [jacs[il][io][:, iu:iu + 1] for il, io, iu in jac_is]
+
+
+def schema_masks(required_ujss, xd):
+    """(d1mask, d2mask) of u~ jets needed by every constraint's requests."""
+    d1 = d2 = 0
+    for req in required_ujss:
+        for iu, ixs in req:
+            if iu != 0:
+                raise FbpxError("the fused engine computes scalar solutions (ud == 1); "
+                                f"request {(iu, ixs)} asks for component {iu}")
+            ixs = tuple(ixs)
+            if any(i < 0 or i >= xd for i in ixs):
+                raise FbpxError(f"derivative index out of range in {(iu, ixs)}")
+            if len(ixs) == 1:
+                d1 |= 1 << ixs[0]
+            elif len(ixs) == 2:
+                if ixs[0] != ixs[1]:
+                    raise FbpxError(f"mixed second derivative {ixs} is not carried by the jet kernels "
+                                    "(diagonal second derivatives only)")
+                d1 |= 1 << ixs[0]
+                d2 |= 1 << ixs[0]
+            elif len(ixs) > 2:
+                raise FbpxError(f"derivative order {len(ixs)} > 2 is not carried by the jet kernels")
+    return d1, d2
+
+
+def _col_index(net):
+    """(kind, dim) -> row of the jet array."""
+    idx = {("v", None): 0}
+    for i in range(net.xd):
+        if (net.d1mask >> i) & 1:
+            idx[("d1", i)] = net.col1(i)
+        if (net.d2mask >> i) & 1:
+            idx[("d2", i)] = net.col2(i)
+    return idx
+
+
+class Epilogue:
+    """Maps kernel jets to the problem loss for one set of training points.
+
+    x          (n, xd) float32 training points (all constraints, concatenated)
+    cranges    [(offset, size)] of each constraint inside x
+    cvalues    per constraint, the list [x_c, *values_c] passed to loss_fn
+    """
+
+    def __init__(self, problem, all_params, x, cranges, cvalues, required_ujss, net):
+        self.problem, self.net = problem, net
+        self.x, self.cranges, self.cvalues = x, cranges, cvalues
+        self.required_ujss = [tuple((iu, tuple(ixs)) for iu, ixs in r) for r in required_ujss]
+        self.jmapss = [get_jmaps(r) for r in self.required_ujss]
+        self.col = _col_index(net)
+        self.affine = self._affine_precompute(all_params)
+
+    # -------------------------------------------------------------- set-up
+
+    def _affine_precompute(self, all_params):
+        C = self.problem.constraining_fn
+        x64 = self.x.detach().double()
+        n = x64.shape[0]
+        if n == 0:
+            return None
+        # independence from trainable parameters
+        probe = _with_grad_trainable(all_params)
+        u0 = torch.zeros((n, 1), dtype=torch.float64, device=x64.device)
+        try:
+            out = C(probe, x64, u0)
+        except Exception:  # noqa: BLE001 - any failure selects the generic path
+            return None
+        if not isinstance(out, torch.Tensor) or out.requires_grad or out.shape != (n, 1):
+            return None
+        # affinity in u
+        g = torch.Generator(device="cpu").manual_seed(1234)
+        r = (torch.rand((n, 1), generator=g, dtype=torch.float64) * 4 - 2).to(x64.device)
+        with torch.no_grad():
+            A = C(all_params, x64, u0)
+            B = C(all_params, x64, u0 + 1) - A
+            Cr = C(all_params, x64, r)
+        scale = 1.0 + A.abs().max() + B.abs().max()
+        if not torch.isfinite(scale) or (Cr - (A + B * r)).abs().max() > 1e-9 * scale:
+            return None
+        # jets of A and B in the schema's columns (float64, nested jvp)
+        req = [(0, ())]
+        for i in range(self.net.xd):
+            if (self.net.d1mask >> i) & 1:
+                req.append((0, (i,)))
+            if (self.net.d2mask >> i) & 1:
+                req.append((0, (i, i)))
+        jm = get_jmaps(tuple(req))
+
+        def fA(xb):
+            return C(all_params, xb, torch.zeros((xb.shape[0], 1), dtype=xb.dtype, device=xb.device)), ()
+
+        def fB(xb):
+            z = torch.zeros((xb.shape[0], 1), dtype=xb.dtype, device=xb.device)
+            return C(all_params, xb, z + 1) - C(all_params, xb, z), ()
+
+        Aj = torch.cat(get_ujs(x64, jm, fA), 1).T.contiguous().float()   # (J, n) in req order
+        Bj = torch.cat(get_ujs(x64, jm, fB), 1).T.contiguous().float()
+        # req order equals the jet column order (value, then per axis d1/d2 interleaved) -> remap
+        order = []
+        for iu, ixs in req:
+            order.append(self.col[("v", None)] if len(ixs) == 0 else
+                         self.col[("d1", ixs[0])] if len(ixs) == 1 else self.col[("d2", ixs[0])])
+        A_cols = torch.empty_like(Aj)
+        B_cols = torch.empty_like(Bj)
+        A_cols[order] = Aj
+        B_cols[order] = Bj
+        return A_cols, B_cols
+
+    # -------------------------------------------------------------- per step
+
+    def _ujs_affine(self, ic, q):
+        A, B = self.affine
+        off, n_c = self.cranges[ic]
+        A = A[:, off:off + n_c]
+        B = B[:, off:off + n_c]
+        out = []
+        for iu, ixs in self.required_ujss[ic]:
+            if len(ixs) == 0:
+                v = A[0] + B[0] * q[0]
+            elif len(ixs) == 1:
+                a = self.col[("d1", ixs[0])]
+                v = A[a] + B[a] * q[0] + B[0] * q[a]
+            else:
+                a = self.col[("d1", ixs[0])]
+                b = self.col[("d2", ixs[0])]
+                v = A[b] + B[b] * q[0] + 2 * B[a] * q[a] + B[0] * q[b]
+            out.append(v[:, None])
+        return out
+
+    def _ujs_generic(self, ic, q, all_params):
+        off, n_c = self.cranges[ic]
+        x0 = self.x[off:off + n_c]
+        net = self.net
+        C = self.problem.constraining_fn
+
+        def u_fn(xb):
+            d = xb - x0
+            ut = q[0]
+            for i in range(net.xd):
+                if (net.d1mask >> i) & 1:
+                    ut = ut + q[net.col1(i)] * d[:, i]
+                if (net.d2mask >> i) & 1:
+                    ut = ut + 0.5 * q[net.col2(i)] * d[:, i] * d[:, i]
+            return C(all_params, xb, ut[:, None]), ()
+        return get_ujs(x0, self.jmapss[ic], u_fn)
+
+    def loss(self, ujet, all_params):
+        """ujet: (J, n) jets of u~ at self.x.  Returns the scalar loss."""
+        constraints_ = []
+        for ic, (off, n_c) in enumerate(self.cranges):
+            q = ujet[:, off:off + n_c]
+            if self.affine is not None:
+                ujs = self._ujs_affine(ic, q)
+            else:
+                ujs = self._ujs_generic(ic, q, all_params)
+            constraints_.append(list(self.cvalues[ic]) + ujs)
+        return self.problem.loss_fn(all_params, constraints_)
+
+    def u(self, ujet, all_params):
+        """Constrained solution value u at self.x (value column only)."""
+        q0 = ujet[0][:, None]
+        return self.problem.constraining_fn(all_params, self.x, q0)
+
+
+def _with_grad_trainable(all_params):
+    """Shallow copy of all_params whose trainable leaves require grad."""
+    def rec(d):
+        if isinstance(d, dict):
+            return {k: rec(v) for k, v in d.items()}
+        if isinstance(d, (list, tuple)):
+            return type(d)(rec(v) for v in d)
+        if isinstance(d, torch.Tensor) and d.is_floating_point():
+            return d.detach().double().requires_grad_(True)
+        return d
+    return {"static": all_params["static"], "trainable": rec(all_params["trainable"])}

@@ -1,0 +1,332 @@
+"""Torch-facing operators over the C ABI of libfbpx.so.
+
+Every operator here is a torch custom op (`torch.ops.fbpx.*`) whose body
+validates device/dtype/contiguity and calls exactly one C-ABI entry point on
+torch's current stream.  Nothing falls back to another implementation: if the
+library is missing or a call fails, FbpxError is raised.
+
+FBPINNJets is the torch.autograd.Function of the path: forward = pair jets +
+sorted POU reduction, backward = POU transpose + the hand-written reverse
+kernel (activation recompute) + deterministic per-subdomain row reduction.
+"""
+
+import ctypes
+from typing import List
+
+import torch
+
+from fbpinns_amd import _lib
+from fbpinns_amd._lib import FBPX_CHUNK, FBPX_DOM_STRIDE, FbpxError, check, lib
+
+__all__ = ["FbpxError", "NetSpec", "Worklist", "FBPINNJets", "FBPX_CHUNK", "FBPX_DOM_STRIDE"]
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t, dtype=None, name="tensor"):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise FbpxError(f"{name} must be a GPU tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise FbpxError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise FbpxError(f"{name} must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class NetSpec:
+    """Network [xd, h x nh, 1] and jet schema (d1mask, d2mask)."""
+
+    def __init__(self, xd, h, nh, d1mask=0, d2mask=0):
+        self.xd, self.h, self.nh, self.d1mask, self.d2mask = int(xd), int(h), int(nh), int(d1mask), int(d2mask)
+        self.J = 1 + bin(self.d1mask).count("1") + bin(self.d2mask).count("1")
+        self.P = _lib.param_count(self.xd, self.h, self.nh)
+
+    def as_list(self):
+        return [self.xd, self.h, self.nh, self.d1mask, self.d2mask]
+
+    def c(self):
+        return _lib.fbpx_net(*self.as_list())
+
+    def with_schema(self, d1mask, d2mask):
+        return NetSpec(self.xd, self.h, self.nh, d1mask, d2mask)
+
+    def supported(self):
+        return _lib.supported(*self.as_list())
+
+    def col1(self, i):
+        return 1 + bin(self.d1mask & ((1 << i) - 1)).count("1")
+
+    def col2(self, i):
+        return 1 + bin(self.d1mask).count("1") + bin(self.d2mask & ((1 << i) - 1)).count("1")
+
+    def __repr__(self):
+        return f"NetSpec(xd={self.xd}, h={self.h}, nh={self.nh}, d1mask={self.d1mask}, d2mask={self.d2mask})"
+
+
+def _net_from_list(net: List[int]):
+    return _lib.fbpx_net(*[int(v) for v in net])
+
+
+# ------------------------------------------------------------------ work list
+
+_WL_TENSORS = ["pair_point", "pair_seg", "chunk_model", "chunk_start", "point_seg_start", "seg_pair_start",
+               "seg_pairs"]
+_WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou"]
+
+
+class Worklist:
+    """Device-resident work list (see fbpx_worklist in include/fbpx.h).
+
+    tensors: dict of int32 CUDA tensors named as in _WL_TENSORS.
+    sizes:   dict of ints named as in _WL_SIZES.
+    """
+
+    def __init__(self, tensors, sizes):
+        self.t = tensors
+        self.n = {k: int(v) for k, v in sizes.items()}
+        for k in _WL_TENSORS:
+            t = self.t[k]
+            if t.dtype != torch.int32 or not t.is_contiguous():
+                raise FbpxError(f"work list tensor {k} must be contiguous int32")
+
+    def tensor_list(self):
+        return [self.t[k] for k in _WL_TENSORS]
+
+    def size_list(self):
+        return [self.n[k] for k in _WL_SIZES]
+
+    def c(self):
+        return _wl_struct(self.tensor_list(), self.size_list())
+
+    def __getattr__(self, k):
+        n = self.__dict__.get("n", {})
+        if k in n:
+            return n[k]
+        raise AttributeError(k)
+
+
+def _wl_struct(ts: List[torch.Tensor], ns: List[int]):
+    s = _lib.fbpx_worklist()
+    for k, v in zip(_WL_SIZES, ns):
+        setattr(s, k, int(v))
+    for k, t in zip(_WL_TENSORS, ts):
+        setattr(s, k, _p(t, torch.int32, k))
+    return s
+
+
+# ------------------------------------------------------------------ custom ops
+
+@torch.library.custom_op("fbpx::inside_count", mutates_args=("counts",))
+def inside_count(x: torch.Tensor, xmin: torch.Tensor, xmax: torch.Tensor, mask: torch.Tensor,
+                 counts: torch.Tensor) -> None:
+    n, xd = x.shape
+    m = xmin.shape[0]
+    mp = _p(mask, torch.uint8, "mask") if mask.numel() else None
+    check(lib().fbpx_inside_count(_stream(), _p(x, torch.float32, "x"), n, xd, _p(xmin, torch.float32, "xmin"),
+                                  _p(xmax, torch.float32, "xmax"), m, mp, _p(counts, torch.int32, "counts")),
+          "fbpx_inside_count")
+
+
+@torch.library.custom_op("fbpx::inside_fill", mutates_args=("n_take", "m_take"))
+def inside_fill(x: torch.Tensor, xmin: torch.Tensor, xmax: torch.Tensor, mask: torch.Tensor,
+                offsets: torch.Tensor, n_take: torch.Tensor, m_take: torch.Tensor) -> None:
+    n, xd = x.shape
+    m = xmin.shape[0]
+    mp = _p(mask, torch.uint8, "mask") if mask.numel() else None
+    check(lib().fbpx_inside_fill(_stream(), _p(x, torch.float32, "x"), n, xd, _p(xmin, torch.float32, "xmin"),
+                                 _p(xmax, torch.float32, "xmax"), m, mp, _p(offsets, torch.int64, "offsets"),
+                                 _p(n_take, torch.int32, "n_take"), _p(m_take, torch.int32, "m_take")),
+          "fbpx_inside_fill")
+
+
+@torch.library.custom_op("fbpx::pou_prepare", mutates_args=("scratch", "wsum"))
+def pou_prepare(net: List[int], dom: torch.Tensor, x: torch.Tensor, wl_t: List[torch.Tensor], wl_n: List[int],
+                scratch: torch.Tensor, wsum: torch.Tensor) -> None:
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    check(lib().fbpx_pou_prepare(_stream(), ctypes.byref(n), _p(dom, torch.float32, "dom"),
+                                 _p(x, torch.float32, "x"), ctypes.byref(w), _p(scratch, torch.float32, "scratch"),
+                                 _p(wsum, torch.float32, "wsum")), "fbpx_pou_prepare")
+
+
+@torch.library.custom_op("fbpx::forward", mutates_args=("pairbuf",))
+def forward(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Tensor, wl_t: List[torch.Tensor],
+            wl_n: List[int], pairbuf: torch.Tensor) -> None:
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    check(lib().fbpx_forward(_stream(), ctypes.byref(n), _p(theta, torch.float32, "theta"),
+                             _p(dom, torch.float32, "dom"), _p(x, torch.float32, "x"), ctypes.byref(w),
+                             _p(pairbuf, torch.float32, "pairbuf")), "fbpx_forward")
+
+
+@torch.library.custom_op("fbpx::pou_reduce", mutates_args=("ujet",))
+def pou_reduce(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf: torch.Tensor,
+               wsum: torch.Tensor, ujet: torch.Tensor) -> None:
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    check(lib().fbpx_pou_reduce(_stream(), ctypes.byref(n), ctypes.byref(w), _p(pairbuf, torch.float32, "pairbuf"),
+                                _p(wsum, torch.float32, "wsum"), _p(ujet, torch.float32, "ujet")),
+          "fbpx_pou_reduce")
+
+
+@torch.library.custom_op("fbpx::pou_backward", mutates_args=("gseg",))
+def pou_backward(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum: torch.Tensor,
+                 g_ujet: torch.Tensor, gseg: torch.Tensor) -> None:
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    check(lib().fbpx_pou_backward(_stream(), ctypes.byref(n), ctypes.byref(w), _p(wsum, torch.float32, "wsum"),
+                                  _p(g_ujet, torch.float32, "g_ujet"), _p(gseg, torch.float32, "gseg")),
+          "fbpx_pou_backward")
+
+
+@torch.library.custom_op("fbpx::backward", mutates_args=("partial",))
+def backward(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Tensor, wl_t: List[torch.Tensor],
+             wl_n: List[int], chunk_train: torch.Tensor, gseg: torch.Tensor, partial: torch.Tensor) -> None:
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    ct = _p(chunk_train, torch.uint8, "chunk_train") if chunk_train.numel() else None
+    check(lib().fbpx_backward(_stream(), ctypes.byref(n), _p(theta, torch.float32, "theta"),
+                              _p(dom, torch.float32, "dom"), _p(x, torch.float32, "x"), ctypes.byref(w), ct,
+                              _p(gseg, torch.float32, "gseg"), _p(partial, torch.float32, "partial")),
+          "fbpx_backward")
+
+
+@torch.library.custom_op("fbpx::reduce_rows", mutates_args=("grad",))
+def reduce_rows(partial: torch.Tensor, rows: torch.Tensor, row_chunk_start: torch.Tensor,
+                grad: torch.Tensor) -> None:
+    P = grad.shape[1]
+    check(lib().fbpx_reduce_rows(_stream(), P, _p(partial, torch.float32, "partial"), _p(rows, torch.int32, "rows"),
+                                 _p(row_chunk_start, torch.int32, "row_chunk_start"), rows.shape[0],
+                                 _p(grad, torch.float32, "grad")), "fbpx_reduce_rows")
+
+
+@torch.library.custom_op("fbpx::adam_rows", mutates_args=("theta", "mu", "nu"))
+def adam_rows(theta: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, grad: torch.Tensor, rows: torch.Tensor,
+              count: torch.Tensor, lr: float, b1: float, b2: float, eps: float) -> None:
+    P = theta.shape[1]
+    check(lib().fbpx_adam_rows(_stream(), _p(theta, torch.float32, "theta"), _p(mu, torch.float32, "mu"),
+                               _p(nu, torch.float32, "nu"), P, _p(grad, torch.float32, "grad"),
+                               _p(rows, torch.int32, "rows"), rows.shape[0], _p(count, torch.int32, "count"),
+                               lr, b1, b2, eps), "fbpx_adam_rows")
+
+
+@torch.library.custom_op("fbpx::adam_flat", mutates_args=("p", "mu", "nu"))
+def adam_flat(p: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, g: torch.Tensor, count: torch.Tensor,
+              lr: float, b1: float, b2: float, eps: float) -> None:
+    check(lib().fbpx_adam_flat(_stream(), _p(p, torch.float32, "p"), _p(mu, torch.float32, "mu"),
+                               _p(nu, torch.float32, "nu"), _p(g, torch.float32, "g"), p.numel(),
+                               _p(count, torch.int32, "count"), lr, b1, b2, eps), "fbpx_adam_flat")
+
+
+@torch.library.custom_op("fbpx::count_increment", mutates_args=("count",))
+def count_increment(count: torch.Tensor) -> None:
+    check(lib().fbpx_count_increment(_stream(), _p(count, torch.int32, "count")), "fbpx_count_increment")
+
+
+@torch.library.custom_op("fbpx::mean_into", mutates_args=("scratch", "out"))
+def mean_into(x: torch.Tensor, scratch: torch.Tensor, out: torch.Tensor) -> None:
+    check(lib().fbpx_mean(_stream(), _p(x, torch.float32, "x"), x.numel(), _p(scratch, torch.float32, "scratch"),
+                          _p(out, torch.float32, "out")), "fbpx_mean")
+
+
+class _Mean(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        xc = x.contiguous()
+        out = torch.empty((), dtype=torch.float32, device=x.device)
+        scratch = torch.empty(256, dtype=torch.float32, device=x.device)
+        mean_into(xc, scratch, out)
+        ctx.shape, ctx.n = x.shape, x.numel()
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        return (g / ctx.n).expand(ctx.shape)
+
+
+def mean(x):
+    """Graph-safe deterministic mean (jnp.mean).  GPU float32 tensors run
+    fbpx_mean; other tensors (CPU set-up work, float64 probes) use torch."""
+    if x.is_cuda and x.dtype == torch.float32 and x.numel() > 0:
+        return _Mean.apply(x)
+    return torch.mean(x)
+
+
+# ------------------------------------------------------------------ take builder
+
+_EMPTY_U8 = {}
+
+
+def _empty_u8(device):
+    if device not in _EMPTY_U8:
+        _EMPTY_U8[device] = torch.empty(0, dtype=torch.uint8, device=device)
+    return _EMPTY_U8[device]
+
+
+def inside_counts(x, xmins, xmaxs, mask=None):
+    """Per-point number of boxes containing the point (GPU)."""
+    counts = torch.empty(x.shape[0], dtype=torch.int32, device=x.device)
+    inside_count(x, xmins, xmaxs, mask if mask is not None else _empty_u8(x.device), counts)
+    return counts
+
+
+def inside_takes(x, xmins, xmaxs, mask=None):
+    """Row-major nonzero of the inside matrix on the GPU: (n_take, m_take)
+    int32 CUDA tensors (decompositions_base.py:70-76)."""
+    counts = inside_counts(x, xmins, xmaxs, mask)
+    offsets = torch.zeros(x.shape[0] + 1, dtype=torch.int64, device=x.device)
+    torch.cumsum(counts, 0, out=offsets[1:])
+    s = int(offsets[-1].item())
+    n_take = torch.empty(s, dtype=torch.int32, device=x.device)
+    m_take = torch.empty(s, dtype=torch.int32, device=x.device)
+    if s:
+        inside_fill(x, xmins, xmaxs, mask if mask is not None else _empty_u8(x.device), offsets, n_take, m_take)
+    return n_take, m_take, counts
+
+
+# ------------------------------------------------------------------ autograd
+
+
+class Plan:
+    """Everything the jets need for one set of takes (built by engine.build_plan)."""
+
+    def __init__(self, net, dom, x, wl, wsum, chunk_train, rows, row_chunk_start, m):
+        self.net, self.dom, self.x, self.wl, self.wsum = net, dom, x, wl, wsum
+        self.chunk_train, self.rows, self.row_chunk_start, self.m = chunk_train, rows, row_chunk_start, m
+        self.pairbuf = torch.empty((max(wl.n_pairs, 1), net.J), dtype=torch.float32, device=x.device)
+        self.partial = torch.empty((max(wl.n_chunks, 1), net.P), dtype=torch.float32, device=x.device)
+        self.gseg = torch.empty((max(wl.n_segs, 1), net.J), dtype=torch.float32, device=x.device)
+
+
+class FBPINNJets(torch.autograd.Function):
+    """ujet[J][n] = jets of the POU-combined subdomain networks at the plan's
+    points (FBPINN_model before constraining_fn, trainers.py:126-170, with the
+    derivatives of _get_ujs, trainers.py:213-247).  Gradient w.r.t. theta
+    ([m][P] packed subdomain rows) for the plan's trained rows only."""
+
+    @staticmethod
+    def forward(ctx, theta, plan):
+        net, wl = plan.net, plan.wl
+        forward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
+        ujet = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=theta.device)
+        pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, ujet)
+        ctx.plan = plan
+        ctx.save_for_backward(theta)
+        return ujet
+
+    @staticmethod
+    def backward(ctx, g_ujet):
+        (theta,) = ctx.saved_tensors
+        plan = ctx.plan
+        net, wl = plan.net, plan.wl
+        g_ujet = g_ujet.contiguous()
+        pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, g_ujet, plan.gseg)
+        backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                 plan.gseg, plan.partial)
+        g_theta = torch.zeros_like(theta)
+        reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
+        return g_theta, None

@@ -1,0 +1,192 @@
+"""Take builder and work-list construction.
+
+get_inputs / get_x_batch / split_takes keep the reference's semantics and
+output format (trainers.py:332-419, 482-507, 544-571) so that the takes are
+bit-exact with the reference's; the O(n m) inside tests run on the GPU
+(fbpx_inside_count / fbpx_inside_fill) and the integer bookkeeping on the
+host in numpy.  build_worklist turns the takes into the kernels' work list
+(include/fbpx.h: fbpx_worklist).
+"""
+
+import numpy as np
+import torch
+
+from fbpinns_amd.ops import FBPX_CHUNK, Worklist
+
+
+def _tree_cut(d, idx):
+    """Cut the leading (subdomain) axis of every leaf under a 'subdomain' key."""
+    def cut(v):
+        if isinstance(v, dict):
+            return {k: cut(x) for k, x in v.items()}
+        if isinstance(v, (list, tuple)):
+            return type(v)(cut(x) for x in v)
+        if isinstance(v, torch.Tensor):
+            return v[torch.as_tensor(idx, dtype=torch.long, device=v.device)]
+        if isinstance(v, np.ndarray):
+            return v[idx]
+        return v
+    return {cl: {k: (cut(v) if k == "subdomain" else v) for k, v in d[cl].items()} for cl in d}
+
+
+def inputs_from_inside(n_take, m_take, training_ims, active, pou_all):
+    """Integer part of get_inputs (trainers.py:347-391) given the inside
+    takes: active remap, all_ims = [active; fixed], m_take re-index and the
+    (point, pou) unique.  Returns takes, all_ims, active, active_ims, fixed_ims."""
+    m = len(pou_all)
+    active = np.array(active, dtype=np.int64).copy()
+    assert np.isin(active, [0, 1, 2]).all()
+    assert active.shape == (m,)
+    active[active == 0] = 1
+    mask = np.zeros_like(active)
+    mask[training_ims] = 1
+    active = active * mask
+    ims_ = np.arange(m)
+    active_ims = ims_[active == 1]
+    fixed_ims = ims_[active == 2]
+    all_ims = np.concatenate([active_ims, fixed_ims])
+    inv = np.zeros(m, dtype=np.int64)
+    inv[all_ims] = np.arange(len(all_ims))
+    m_take = inv[m_take]
+    pou_all = np.asarray(pou_all).astype(np.int64).reshape(-1)
+    pous = pou_all[all_ims]
+    # jnp.unique over (point, pou) rows (lexicographic order) with inverse,
+    # through the equivalent 1-D key point * npou_max + pou
+    pt = pous[m_take]
+    npou_max = int(pou_all.max()) + 1 if len(pou_all) else 1
+    ukey, p_take = np.unique(n_take * npou_max + pt, return_inverse=True)
+    p_take = p_take.reshape(-1)
+    np_take = ukey // npou_max
+    npou = len(np.unique(pou_all))
+    return (m_take, n_take, p_take, np_take, npou), all_ims, active, active_ims, fixed_ims
+
+
+def get_inputs(x_batch, active, all_params, decomposition):
+    """trainers.py:332-419.  x_batch: (n, xd) float32 GPU tensor.
+    Returns takes = (m_take, n_take, p_take, np_take, npou) as int64 numpy
+    arrays (m_take indexes all_ims), all_ims, and
+    (active, cut_active, cut_fixed, cut_all, merge_active)."""
+    n_take, m_take, training_ims = decomposition.inside_points(all_params, x_batch)
+    pou_all = np.asarray(all_params["static"]["decomposition"]["subdomain"]["pou"].cpu()).reshape(-1)
+    takes, all_ims, active, active_ims, fixed_ims = inputs_from_inside(n_take, m_take, training_ims, active,
+                                                                      pou_all)
+
+    def cut_active(d):
+        return _tree_cut(d, active_ims)
+
+    def cut_fixed(d):
+        return _tree_cut(d, fixed_ims)
+
+    def cut_all(d):
+        return _tree_cut(d, all_ims)
+
+    def merge_active(da, d):
+        for cl in d:
+            for k in d[cl]:
+                if k == "subdomain":
+                    d[cl][k] = _merge(da[cl][k], d[cl][k], active_ims)
+                else:
+                    d[cl][k] = da[cl][k]
+        return d
+
+    return takes, all_ims, (active, cut_active, cut_fixed, cut_all, merge_active)
+
+
+def _merge(va, v, idx):
+    if isinstance(v, dict):
+        return {k: _merge(va[k], v[k], idx) for k in v}
+    if isinstance(v, (list, tuple)):
+        return type(v)(_merge(a, b, idx) for a, b in zip(va, v))
+    out = v.clone()
+    out[torch.as_tensor(idx, dtype=torch.long, device=v.device)] = va
+    return out
+
+
+def get_x_batch(active, all_params, x_batch_global, constraint_offsets, constraint_sizes, decomposition):
+    """trainers.py:482-507 index bookkeeping: training_ips (points inside >= 1
+    strictly active model), constraint_fs (n_train, c), constraint_ips and d."""
+    active = np.asarray(active)
+    ims = np.arange(all_params["static"]["decomposition"]["m"])[active == 1]
+    training_ips, d = decomposition.inside_models(all_params, x_batch_global, ims)
+    n_glob = x_batch_global.shape[0]
+    fs_global = np.zeros((n_glob, len(constraint_sizes)), dtype=bool)
+    for ic, (o, s) in enumerate(zip(constraint_offsets, constraint_sizes)):
+        fs_global[o:o + s, ic] = True
+    constraint_fs = fs_global[training_ips]
+    ix_ = np.arange(len(training_ips))
+    constraint_ips = [ix_[f] for f in constraint_fs.T]
+    return training_ips, constraint_fs, constraint_ips, d
+
+
+def split_takes(takes, constraint_fs, constraint_ips, n_train):
+    """trainers.py:544-571: per-constraint re-indexed takes."""
+    m_take, n_take, p_take, np_take, npou = takes
+    takess = []
+    iu_ = np.arange(np_take.shape[0])
+    for f, ips in zip(constraint_fs.T, constraint_ips):
+        f2 = f[np_take]
+        ius = iu_[f2]
+        f3 = f[n_take]
+        f4 = f2[p_take]
+        inv = np.zeros(n_train, dtype=np.int64)
+        inv[ips] = np.arange(len(ips))
+        inv2 = np.zeros(np_take.shape[0], dtype=np.int64)
+        inv2[ius] = np.arange(len(ius))
+        takess.append((m_take[f3], inv[n_take[f3]], inv2[p_take[f4]], inv[np_take[f2]], npou))
+    return takess
+
+
+class WorkPlan:
+    """Work list plus the row bookkeeping of one set of takes."""
+
+    def __init__(self, wl, chunk_train, rows, row_chunk_start, n_active, all_ims):
+        self.wl, self.chunk_train, self.rows, self.row_chunk_start = wl, chunk_train, rows, row_chunk_start
+        self.n_active, self.all_ims = n_active, all_ims
+
+
+def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK):
+    """Kernel work list from the reference-format takes.
+
+    Tile order groups the pairs by position in all_ims = [active; fixed]
+    (stable, so ascending point order inside a subdomain); each subdomain's
+    run is cut into chunks of <= `chunk` pairs.  Segments are the unique
+    (point, pou) rows; seg_pairs lists each segment's pairs (tile-order ids)
+    in the reference's pair order."""
+    m_take, n_take, p_take, np_take, npou = takes
+    s = len(m_take)
+    n_all = len(all_ims)
+    order = np.argsort(m_take, kind="stable")
+    counts = np.bincount(m_take, minlength=n_all)
+    nch = (counts + chunk - 1) // chunk
+    chunk_pos = np.repeat(np.arange(n_all), nch)                    # all_ims position of each chunk
+    first = np.concatenate([[0], np.cumsum(nch)])[:-1]              # first chunk of each position
+    model_start = np.concatenate([[0], np.cumsum(counts)])
+    within = np.arange(len(chunk_pos)) - first[chunk_pos]
+    chunk_start = np.concatenate([model_start[chunk_pos] + within * chunk, [s]]).astype(np.int32)
+    chunk_model = np.asarray(all_ims, dtype=np.int64)[chunk_pos].astype(np.int32)
+    chunk_train = (chunk_pos < n_active).astype(np.uint8)
+    row_chunk_start = np.concatenate([first[:n_active], [first[n_active] if n_active < n_all else len(chunk_pos)]])
+    rows = np.asarray(all_ims[:n_active], dtype=np.int32)
+
+    tile_pos = np.empty(s, dtype=np.int64)
+    tile_pos[order] = np.arange(s)
+    n_segs = len(np_take)
+    seg_order = np.argsort(p_take, kind="stable")
+    seg_pairs = tile_pos[seg_order].astype(np.int32)
+    seg_pair_start = np.concatenate([[0], np.cumsum(np.bincount(p_take, minlength=n_segs))]).astype(np.int32)
+    point_seg_start = np.searchsorted(np_take, np.arange(n_points + 1), side="left").astype(np.int32)
+
+    dev = lambda a, dt=torch.int32: torch.as_tensor(np.ascontiguousarray(a)).to(device=device, dtype=dt)
+    tensors = {
+        "pair_point": dev(n_take[order]),
+        "pair_seg": dev(p_take[order]),
+        "chunk_model": dev(chunk_model),
+        "chunk_start": dev(chunk_start),
+        "point_seg_start": dev(point_seg_start),
+        "seg_pair_start": dev(seg_pair_start),
+        "seg_pairs": dev(seg_pairs),
+    }
+    sizes = {"n_pairs": s, "n_chunks": len(chunk_pos), "n_points": n_points, "n_segs": n_segs, "npou": npou}
+    wl = Worklist(tensors, sizes)
+    return WorkPlan(wl, dev(chunk_train, torch.uint8), dev(rows), dev(row_chunk_start), n_active,
+                    np.asarray(all_ims))

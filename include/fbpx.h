@@ -1,0 +1,187 @@
+/*
+ * fbpx.h -- C ABI of libfbpx.so, the MI355X (gfx950) FBPINN train-step engine.
+ *
+ * Every pointer is a DEVICE pointer owned by the caller unless stated.  No
+ * entry point allocates device memory, synchronises the stream or calls back
+ * into the host, so a sequence of calls can be captured into a hipGraph.
+ * Every call returns 0 on success and a negative code on error; the text of
+ * the last error on the calling thread is returned by fbpx_last_error().
+ * Nothing throws across the ABI.  `stream` is a hipStream_t passed as void*.
+ *
+ * The derivative jet of a quantity q at a point is stored as J contiguous
+ * floats: [q, dq/dx_i for each i in d1mask (ascending i),
+ *          d2q/dx_i^2 for each i in d2mask (ascending i)],
+ * J = 1 + popcount(d1mask) + popcount(d2mask), d2mask a subset of d1mask.
+ *
+ * Subdomain parameter rows (theta, [m][P] floats) pack the FCN layers of one
+ * subdomain in the reference's layout (networks.py:50-58: W is (out, in),
+ * row-major): W1 (h*xd), b1 (h), [W2 (h*h), b2 (h)] if nh == 2, WL (h), bL (1).
+ *
+ * Subdomain static rows (dom, [m][FBPX_DOM_STRIDE] floats):
+ *   xmin[0..3], xmax[0..3], unnorm_mu, unnorm_sd, flag, pad
+ * (decompositions.py:127-128,181: xmins, xmaxs, flags, unnorms).
+ */
+#ifndef FBPX_H
+#define FBPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBPX_DOM_STRIDE 12
+#define FBPX_MAX_XD 3
+
+enum {
+  FBPX_OK = 0,
+  FBPX_EINVAL = -1,      /* bad argument (size, null pointer, mask) */
+  FBPX_EUNSUPPORTED = -2,/* network / jet shape not compiled in */
+  FBPX_EHIP = -3         /* HIP runtime error at launch */
+};
+
+/* Network and jet schema.  Layer sizes are [xd, h x nh, 1]. */
+typedef struct {
+  int32_t xd;      /* input dimension, 1..3 */
+  int32_t h;       /* hidden width: 16, 32 or 64 */
+  int32_t nh;      /* number of hidden layers: 1 or 2 */
+  int32_t d1mask;  /* bit i set: first derivative along x_i is carried */
+  int32_t d2mask;  /* bit i set: d2/dx_i^2 is carried (subset of d1mask) */
+} fbpx_net;
+
+/*
+ * Work list for one set of takes.  Pairs are (point, subdomain) memberships.
+ * "Tile order": pairs grouped by subdomain (ascending chunk index), within a
+ * subdomain in ascending point order.  Each chunk is a contiguous run of at
+ * most FBPX_CHUNK pairs of ONE subdomain.  "Segment": a unique
+ * (point, pou) row of the reference's jnp.unique (trainers.py:380-385),
+ * ordered by (point, pou).
+ */
+#define FBPX_CHUNK 256
+typedef struct {
+  int32_t n_pairs;
+  int32_t n_chunks;
+  int32_t n_points;
+  int32_t n_segs;
+  int32_t npou;
+  const int32_t* pair_point;      /* [n_pairs] tile order -> point index */
+  const int32_t* pair_seg;        /* [n_pairs] tile order -> segment index */
+  const int32_t* chunk_model;     /* [n_chunks] subdomain row of the chunk */
+  const int32_t* chunk_start;     /* [n_chunks+1] tile-order start, last = n_pairs */
+  const int32_t* point_seg_start; /* [n_points+1] segments of each point */
+  const int32_t* seg_pair_start;  /* [n_segs+1] into seg_pairs */
+  const int32_t* seg_pairs;       /* [n_pairs] tile-order pair ids grouped by segment */
+} fbpx_worklist;
+
+/* ---------------------------------------------------------------- takes */
+
+/* Number of boxes containing each point (inclusive test xmin <= x <= xmax on
+ * every axis, FP32), over models with model_mask[j] != 0 (model_mask may be
+ * NULL = all).  Replaces _inside_sum_batch (decompositions_base.py:23-48) as
+ * used by inside_points / inside_models (decompositions.py:201-215,217-227).
+ * x: [n][xd]; xmin, xmax: [m][xd]; counts: [n] int32. */
+int fbpx_inside_count(void* stream, const float* x, int32_t n, int32_t xd,
+                      const float* xmin, const float* xmax, int32_t m,
+                      const uint8_t* model_mask, int32_t* counts);
+
+/* Row-major nonzero of the inside matrix: for point p, the ids of the boxes
+ * containing it, ascending, at offsets[p]..offsets[p+1].  Bit-exact with
+ * _inside_take_batch (decompositions_base.py:50-68, order pinned by its
+ * self-test :100-128).  offsets: [n+1] int64 exclusive scan of counts. */
+int fbpx_inside_fill(void* stream, const float* x, int32_t n, int32_t xd,
+                     const float* xmin, const float* xmax, int32_t m,
+                     const uint8_t* model_mask, const int64_t* offsets,
+                     int32_t* n_take, int32_t* m_take);
+
+/* ---------------------------------------------------------------- forward */
+
+/* Window jets summed per segment: wsum[seg][J] = sum over the segment's pairs
+ * of the jets of window_fn (decompositions.py:196-199, windows.py:25-35).
+ * Parameter-independent; computed once per work list.  pair_scratch:
+ * [n_pairs][J] caller-provided work space (e.g. the forward pair buffer). */
+int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom,
+                     const float* x, const fbpx_worklist* wl, float* pair_scratch,
+                     float* wsum);
+
+/* Per-pair jets of u*w, where u = unnorm(FCN(norm(x))) and w = window(x),
+ * i.e. the vmapped FBPINN_model_inner (trainers.py:113-118,158) with the
+ * nested-jvp tangents of _get_ujs (trainers.py:213-247) carried in registers.
+ * theta: [m][P]; pairbuf: [n_pairs][J] (tile order). */
+int fbpx_forward(void* stream, const fbpx_net* net, const float* theta,
+                 const float* dom, const float* x, const fbpx_worklist* wl,
+                 float* pairbuf);
+
+/* POU reduction: per point, sum over pou of (sum_pairs u*w) / (sum_pairs w)
+ * with the quotient rule on jets, divided by npou (trainers.py:162-170).
+ * Sorted, atomic-free.  ujet: [J][n_points] (column-major jets). */
+int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
+                    const float* pairbuf, const float* wsum, float* ujet);
+
+/* ---------------------------------------------------------------- backward */
+
+/* Transpose of fbpx_pou_reduce: from d loss / d ujet ([J][n_points]) to
+ * d loss / d (segment sum of u*w jets), gseg: [n_segs][J]. */
+int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
+                      const float* wsum, const float* g_ujet, float* gseg);
+
+/* Reverse pass through the jets to the subdomain weights (the part of
+ * value_and_grad, trainers.py:292, below the segment sums).  Recomputes
+ * activations.  Chunks with chunk_train[c] == 0 (fixed subdomains,
+ * trainers.py:252-256 -- no gradient) are skipped and their partial row is
+ * left untouched.  partial: [n_chunks][P] per-chunk gradient rows. */
+int fbpx_backward(void* stream, const fbpx_net* net, const float* theta,
+                  const float* dom, const float* x, const fbpx_worklist* wl,
+                  const uint8_t* chunk_train, const float* gseg, float* partial);
+
+/* Per-row gradient from the per-chunk partial rows: for k < n_rows,
+ *   grad[rows[k]][e] = sum over c in row_chunk_start[k] .. row_chunk_start[k+1]-1
+ *                      (ascending) of partial[c][e].
+ * Rows not listed are not written.  grad: [m][P]. */
+int fbpx_reduce_rows(void* stream, int32_t P, const float* partial,
+                     const int32_t* rows, const int32_t* row_chunk_start,
+                     int32_t n_rows, float* grad);
+
+/* ---------------------------------------------------------------- Adam */
+
+/* optax.adam on subdomain rows (trainers.py:294-295), for each listed row r
+ * (rows[k], k < n_rows) with gradient grad[r]:
+ *   mu = (1-b1) g + b1 mu;  nu = (1-b2) g^2 + b2 nu;
+ *   p += -lr * (mu/(1-b1^c)) / (sqrt(nu/(1-b2^c)) + eps),  c = *count + 1.
+ * Rows not listed (fixed or discarded subdomains) are untouched.
+ * theta, mu, nu, grad: [m][P].  The hyper-parameters are doubles so that
+ * f32(1 - b1) and f32(-lr) are formed as the reference forms them (Python
+ * floats meeting FP32 arrays). */
+int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu,
+                   int32_t P, const float* grad, const int32_t* rows, int32_t n_rows,
+                   const int32_t* count, double lr, double b1, double b2, double eps);
+
+/* The same update on a flat vector p[n] with gradient g[n] (the non-subdomain
+ * trainable leaves, e.g. a learnable PDE coefficient, problems.py:232-234). */
+int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g,
+                   int32_t n, const int32_t* count, double lr, double b1, double b2,
+                   double eps);
+
+/* *count += 1 (optax's shared int32 step count, trainers.py:531,640). */
+int fbpx_count_increment(void* stream, int32_t* count);
+
+/* Deterministic FP32 mean of x[n] into out[0] (jnp.mean in the problems'
+ * loss_fn, problems.py:123,204,265,326,405).  Two fixed-shape stages with
+ * an LDS tree, no atomics and no memset (safe inside a captured graph).
+ * scratch: [256] floats of caller work space. */
+int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out);
+
+/* ---------------------------------------------------------------- misc */
+
+/* Parameters per subdomain row for a network shape (0 if unsupported). */
+int32_t fbpx_param_count(const fbpx_net* net);
+/* 1 if kernels for this network/jet shape are compiled in. */
+int32_t fbpx_supported(const fbpx_net* net);
+/* Text of the last error on this thread ("" if none). */
+const char* fbpx_last_error(void);
+/* Library version string. */
+const char* fbpx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FBPX_H */

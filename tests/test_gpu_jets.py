@@ -1,0 +1,142 @@
+"""GPU parity: take builder, forward jets, POU reduction and the backward
+kernel against the float64 oracle (nested jvp / autograd) on identical
+float32 inputs.
+
+Tolerances (stated per check):
+  takes: bit-exact (integer work).
+  jets:  per jet column, max |gpu - oracle| <= 1e-5 * max |oracle|.
+  grads: per layer tensor, max |gpu - oracle| <= 1e-4 * max |oracle|.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from fbpinns_amd import ops
+from fbpinns_amd.engine import FusedEngine
+from fbpinns_amd.ops import FBPINNJets
+from fbpinns_amd.problems import Problem
+
+import setups
+from oracle import model as om
+
+pytestmark = pytest.mark.gpu
+
+JET_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+# name, nsub, lo, hi, overlap, layer_sizes, schema (d1, d2), points, multilevel
+CASES = [
+    ("ho_1d_h32", (15,), (0.0,), (1.0,), 2.1, [1, 32, 1], (1, 1), ("grid", (200,)), False),
+    ("1d_h16_nh2_first", (6,), (0.0,), (1.0,), 2.5, [1, 16, 16, 1], (1, 0), ("uniform", 500), False),
+    ("poisson_2d_h32", (4, 4), (0.0, 0.0), (1.0, 1.0), 1.9, [2, 32, 32, 1], (3, 3), ("uniform", 3000), False),
+    ("burgers_2d_h32", (5, 3), (-1.0, 0.0), (1.0, 1.0), 2.9, [2, 32, 32, 1], (3, 1), ("uniform", 3000), False),
+    ("2d_h64_nh1", (3, 3), (0.0, 0.0), (1.0, 1.0), 1.9, [2, 64, 1], (3, 3), ("uniform", 2000), False),
+    ("wave_3d_h64", (3, 3, 2), (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0), 1.9, [3, 64, 64, 1], (7, 7),
+     ("uniform", 3000), False),
+    ("3d_h32_value", (3, 2, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2.0, [3, 32, 32, 1], (0, 0),
+     ("uniform", 2000), False),
+    ("multilevel_1d", (5,), (0.0,), (1.0,), 2.3, [1, 32, 32, 1], (1, 1), ("uniform", 400), True),
+]
+
+
+def _setup(case, seed=0):
+    name, nsub, lo, hi, ov, ls, (d1, d2), (kind, npts), ml = case
+    dec_cls, ap = setups.build(nsub, lo, hi, ov, ls, seed=seed, multilevel=ml, unnorm=(0.1, 1.3))
+    x = setups.grid_points(npts, lo, hi) if kind == "grid" else setups.uniform_points(npts, lo, hi, seed=seed + 7)
+    eng = FusedEngine(Problem, dec_cls, __import__("fbpinns_amd.networks").networks.FCN, ap, device="cuda:0",
+                      graph=False)
+    return eng, ap, x, len(lo), d1, d2
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_takes_bit_exact(case):
+    eng, ap, x, xd, d1, d2 = _setup(case)
+    print(f"[takes] {case[0]} n={x.shape[0]}", flush=True)
+    plan, takes = eng.make_plan(x.cuda(), schema=(d1, d2))
+    otakes, oall, *_ = setups.oracle_view(ap, x)
+    for a, b, nm in zip(takes[:4], otakes[:4], ["m_take", "n_take", "p_take", "np_take"]):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), nm
+    assert takes[4] == otakes[4]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_forward_jets(case):
+    eng, ap, x, xd, d1, d2 = _setup(case)
+    print(f"[fwd] {case[0]} n={x.shape[0]} schema=({d1},{d2})", flush=True)
+    plan, _ = eng.make_plan(x.cuda(), schema=(d1, d2))
+    with torch.no_grad():
+        ujet = FBPINNJets.apply(eng.theta, plan)
+    torch.cuda.synchronize()
+    ref, _, _ = setups.oracle_jets(ap, x, xd, d1, d2)
+    err = setups.col_rel_err(ujet, ref)
+    print("column errors", err.tolist())
+    assert torch.isfinite(ujet).all()
+    assert (err <= JET_TOL).all(), err
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_backward_grads(case):
+    eng, ap, x, xd, d1, d2 = _setup(case, seed=3)
+    print(f"[bwd] {case[0]} n={x.shape[0]} schema=({d1},{d2})", flush=True)
+    plan, _ = eng.make_plan(x.cuda(), schema=(d1, d2))
+    J, n = plan.net.J, x.shape[0]
+    r = torch.tensor(np.random.default_rng(5).standard_normal((J, n)), dtype=torch.float32)
+    th = eng.theta.detach().requires_grad_(True)
+    ujet = FBPINNJets.apply(th, plan)
+    (ujet * r.cuda()).sum().backward()
+    g = th.grad.detach().cpu()
+    # oracle: same synthetic loss, float64 autograd through the nested jvp
+    takes, all_ims, layers, dom, _ = setups.oracle_view(ap, x)
+    jm = om.get_jmaps(setups.schema_request(xd, d1, d2))
+    layers = [(W.requires_grad_(True), b.requires_grad_(True)) for W, b in layers]
+    ujs = om.fbpinn_forward(layers, dom, x.double(), takes, om.identity_constraint, jm)
+    L = (torch.cat(ujs, 1).T * r.double()).sum()
+    flat = [t for W, b in layers for t in (W, b)]
+    og = torch.autograd.grad(L, flat)
+    off = 0
+    for (W, b), gW, gb in zip(layers, og[0::2], og[1::2]):
+        o, i = W.shape[1], W.shape[2]
+        kW = g[all_ims, off:off + o * i].reshape(len(all_ims), o, i)
+        off += o * i
+        kb = g[all_ims, off:off + o]
+        off += o
+        for a, bref, nm in ((kW, gW, "W"), (kb, gb, "b")):
+            e = ((a.double() - bref).abs().max() / bref.abs().max().clamp_min(1e-30)).item()
+            print(f"  {nm}{tuple(bref.shape)} rel err {e:.2e}")
+            assert e <= GRAD_TOL, (nm, e)
+    unused = np.setdiff1d(np.arange(eng.m), all_ims)
+    assert (g[unused] == 0).all()
+
+
+def test_fixed_subdomains_get_no_gradient():
+    case = CASES[2]
+    eng, ap, x, xd, d1, d2 = _setup(case)
+    m = eng.m
+    active = np.ones(m, dtype=int)
+    active[::3] = 2
+    print(f"[fixed] m={m} fixed={int((active == 2).sum())}", flush=True)
+    plan, takes = eng.make_plan(x.cuda(), active=active, schema=(d1, d2))
+    th = eng.theta.detach().requires_grad_(True)
+    ujet = FBPINNJets.apply(th, plan)
+    r = torch.ones_like(ujet)
+    (ujet * r).sum().backward()
+    g = th.grad.detach().cpu()
+    assert (g[active == 2] == 0).all()
+    assert (g[active == 1].abs().sum(1) > 0).all()
+
+
+def test_inside_empty_and_edges():
+    """Points exactly on box edges are inside (inclusive); no points / no boxes."""
+    xmins = torch.tensor([[0.0], [0.5]], device="cuda")
+    xmaxs = torch.tensor([[0.5], [1.0]], device="cuda")
+    x = torch.tensor([[0.0], [0.5], [1.0], [1.5], [-0.1]], device="cuda")
+    n_take, m_take, counts = ops.inside_takes(x, xmins, xmaxs)
+    assert n_take.tolist() == [0, 1, 1, 2]
+    assert m_take.tolist() == [0, 0, 1, 1]
+    assert counts.tolist() == [1, 2, 1, 0, 0]
+    e = torch.empty((0, 1), device="cuda")
+    n2, m2, c2 = ops.inside_takes(e, xmins, xmaxs)
+    assert n2.numel() == 0 and c2.numel() == 0
+    n3, m3, c3 = ops.inside_takes(x, xmins[:0], xmaxs[:0])
+    assert n3.numel() == 0 and c3.tolist() == [0] * 5
