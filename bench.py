@@ -1,0 +1,251 @@
+"""Benchmark: FBPINN train steps/s (and collocation points/s) of the fused,
+graph-captured step on one MI355X, against the plain per-op GPU form of the
+same method and a bounded CPU sample of the oracle restatement.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config wave]
+
+Prints ONE JSON line.  Default workload: the (2+1)D wave configuration
+(1,048,576 collocation points, 12x12x6 subdomains, overlap 1.9, [3,64,64,1]).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_MFMA = 157.3   # TFLOP/s, MI355X dense FP32 matrix
+PEAK_HBM = 8000.0         # GB/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_kernel(fn, reps):
+    """Average duration (ms) of fn() launched on torch's current stream,
+    measured with HIP events on that stream."""
+    s = torch.cuda.current_stream()
+    fn()
+    s.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def kernel_roofline(eng, cfgmod):
+    from fbpinns_amd import ops
+    plan, net = eng.plan, eng.net
+    wl = plan.wl
+    # fill the backward inputs once (gseg) with a real step's gradient
+    loss, g, _ = eng.loss_and_grad()
+    torch.cuda.synchronize()
+
+    def fwd():
+        ops.forward(net.as_list(), eng.theta, eng.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
+
+    def bwd():
+        ops.backward(net.as_list(), eng.theta, eng.dom, plan.x, wl.tensor_list(), wl.size_list(),
+                     plan.chunk_train, plan.gseg, plan.partial)
+
+    t_fwd = time_kernel(fwd, 10)
+    t_bwd = time_kernel(bwd, 10)
+    f_pair = cfgmod.fwd_flops_per_pair(net.xd, net.h, net.nh, net.J)
+    trained_pairs = int((plan.chunk_train.cpu().numpy().astype(bool)
+                         * np.diff(wl.t["chunk_start"].cpu().numpy())).sum())
+    fl_fwd = f_pair * wl.n_pairs
+    fl_bwd = 2 * f_pair * trained_pairs
+    return dict(t_fwd_ms=t_fwd, t_bwd_ms=t_bwd, flop_fwd=fl_fwd, flop_bwd=fl_bwd,
+                tflops_fwd=fl_fwd / (t_fwd * 1e-3) / 1e12, tflops_bwd=fl_bwd / (t_bwd * 1e-3) / 1e12,
+                f_pair=f_pair, n_pairs=wl.n_pairs, trained_pairs=trained_pairs)
+
+
+def cpu_baseline(c, eng, budget_s=20.0):
+    """The oracle (torch on CPU, float32) on a bounded sample of the same
+    points, all subdomains; returns steps/s extrapolated by pair count."""
+    from oracle import adam as oadam
+    from oracle import model as om
+    from oracle import problems as opb
+    from oracle import takes as ot
+    torch.set_num_threads(min(os.cpu_count() or 1, 16))
+    cores = torch.get_num_threads()
+    ap = eng.all_params
+    p = ap["static"]["decomposition"]["subdomain"]["params"]
+    xmins, xmaxs = p[0].cpu().numpy(), p[1].cpu().numpy()
+    pou = ap["static"]["decomposition"]["subdomain"]["pou"].cpu().numpy().reshape(-1)
+    prob = ap["static"]["problem"]
+    name = c.problem.__name__
+    if name == "WaveEquationConstantVelocity3D":
+        req, C, L = opb.wave(prob["source"].cpu().numpy(), prob["c0"])
+    elif name == "PoissonMultiscale2D":
+        req, C, L = opb.poisson(prob["sd"], prob["ks"])
+    elif name == "BurgersEquation2D":
+        req, C, L = opb.burgers(prob["nu"], prob["sd"])
+    elif name == "HarmonicOscillator1DHardBC":
+        req, C, L = opb.ho_hard(prob["d"], prob["w0"], prob["sd"])
+    else:
+        return None
+    xall = eng.x.cpu().numpy()
+    n_full = xall.shape[0]
+    n = min(n_full, 4096)
+    while True:
+        x = xall[:n]
+        takes, all_ims, *_ = ot.get_inputs(x, np.ones(len(pou), dtype=int), xmins, xmaxs, pou)
+        layers = [(W.detach().cpu()[all_ims].float(), b.detach().cpu()[all_ims].float())
+                  for W, b in ap["trainable"]["network"]["subdomain"]["layers"]]
+        dom = om.to_torch_dom(xmins[all_ims], xmaxs[all_ims], p[4].cpu().numpy()[all_ims],
+                              p[5].cpu().numpy()[all_ims], torch.float32)
+        jm = [om.get_jmaps(r) for r in req]
+        cons = [[torch.tensor(x)]]
+
+        def one_step(layers):
+            loss, g, _ = om.value_and_grad(layers, dom, cons, [takes], jm, C, L)
+            out = []
+            for (W, b), (gW, gb) in zip(layers, g):
+                Wn, *_ = oadam.adam_update(W.numpy(), np.zeros_like(W.numpy()), np.zeros_like(W.numpy()),
+                                           gW.numpy(), 0)
+                bn, *_ = oadam.adam_update(b.numpy(), np.zeros_like(b.numpy()), np.zeros_like(b.numpy()),
+                                           gb.numpy(), 0)
+                out.append((torch.tensor(Wn), torch.tensor(bn)))
+            return out
+        t0 = time.time()
+        layers = one_step(layers)
+        t1 = time.time() - t0
+        if t1 * 3 > budget_s or n == n_full:
+            break
+        n = min(n_full, int(n * max(2.0, min(8.0, budget_s / (3 * max(t1, 1e-3))))))
+    steps = max(1, min(3, int(budget_s / max(t1, 1e-3))))
+    t0 = time.time()
+    for _ in range(steps):
+        layers = one_step(layers)
+    dt = (time.time() - t0) / steps
+    s_sample = len(takes[0])
+    full_pairs = eng.plan.wl.n_pairs
+    rate = (s_sample / dt) / full_pairs
+    return {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (torch CPU float32, nested jvp + autograd + Adam) on the first {n} of {n_full} "
+                      f"points ({s_sample} of {full_pairs} pairs), {steps} timed steps of {dt:.2f} s; "
+                      f"steps/s extrapolated by pair count"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="wave")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-chunks", type=int, default=16)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every run here uses one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fbpinns_amd import configs
+    from fbpinns_amd.plain import PlainFBPINN
+    from fbpinns_amd.trainers import FBPINNTrainer
+    from fbpinns_amd.util.logger import logger
+    for h in logger.handlers:
+        h.setStream(sys.stderr)   # stdout carries only the JSON line
+
+    c = configs.CONFIGS[a.config]()
+    log(f"config {a.config}: building problem")
+    tr = FBPINNTrainer(c, write_outputs=False)
+    eng, sched, x_test, _ = tr.setup(exact=False)
+    active = next(iter(sched))
+    t0 = time.time()
+    eng.set_active(active)
+    torch.cuda.synchronize()
+    log(f"work list + graph capture {time.time() - t0:.1f} s: {eng.plan.wl.n_points} points, "
+        f"{eng.plan.wl.n_pairs} pairs")
+    for _ in range(a.warmup):
+        eng.step()
+    torch.cuda.synchronize()
+    log("timing fused steps")
+    t0 = time.time()
+    for _ in range(a.steps):
+        loss = eng.step()
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    sps = a.steps / dt
+    n_pts = eng.plan.wl.n_points
+    log(f"fused: {sps:.2f} steps/s, loss {loss.item():.4e}")
+
+    rl = kernel_roofline(eng, configs)
+    log(f"kernels: fwd {rl['t_fwd_ms']:.3f} ms ({rl['tflops_fwd']:.1f} TFLOP/s), "
+        f"bwd {rl['t_bwd_ms']:.3f} ms ({rl['tflops_bwd']:.1f} TFLOP/s)")
+    dom_is_bwd = rl["t_bwd_ms"] >= rl["t_fwd_ms"]
+    ach = rl["tflops_bwd"] if dom_is_bwd else rl["tflops_fwd"]
+
+    plain = None
+    if not a.no_plain:
+        log("plain per-op form")
+        eng2 = FBPINNTrainer(configs.CONFIGS[a.config](graph=False), write_outputs=False).setup(exact=False)[0]
+        pl = PlainFBPINN(c.problem, c.decomposition, c.network, eng2,
+                         point_chunks=a.plain_chunks if len(eng2.cglobal) == 1 else 1)
+        pl.set_active(active)
+        eng2.set_active(active)
+        l_plain0 = pl.step().item()
+        l_fused0 = eng2.step().item()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(a.plain_steps):
+            pl.step()
+        torch.cuda.synchronize()
+        pdt = (time.time() - t0) / a.plain_steps
+        plain = {"steps_per_s": 1.0 / pdt, "ms_per_step": pdt * 1e3, "point_chunks": pl.point_chunks,
+                 "first_step_loss_rel_diff_vs_fused": abs(l_plain0 - l_fused0) / abs(l_fused0)}
+        log(f"plain: {1.0 / pdt:.4f} steps/s; speed-up {sps * pdt:.1f}x")
+        del pl, eng2
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline (oracle sample)")
+        cpu = cpu_baseline(c, eng, a.cpu_budget)
+
+    out = {
+        "metric": "FBPINN train steps/s + collocation pts/s at matched L1 error, fused vs plain",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded collocation points, random-init subdomain networks)",
+        "config": {"workload": f"{a.config}: {n_pts} collocation points, {eng.m} subdomains, "
+                               f"{eng.plan.wl.n_pairs} (point, subdomain) pairs, FCN [{eng.net.xd}, "
+                               f"{eng.net.h}x{eng.net.nh}, 1], jets J={eng.net.J}, hipGraph-captured step",
+                   "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs},
+        "collocation_pts_per_s": sps * n_pts,
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP32_MFMA, "traffic": None,
+                     "kernel": "bwd_kernel" if dom_is_bwd else "fwd_kernel"},
+        "kernels": rl,
+        "plain_gpu": plain,
+        "speedup_vs_plain": (sps / plain["steps_per_s"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -230,6 +230,21 @@ class FusedEngine:
 
     # -------------------------------------------------------------- step
 
+    def loss_and_grad(self):
+        """Loss and gradients at the current parameters without an update:
+        (loss, d loss/d theta [m][P], d loss/d pflat [Q] or None)."""
+        plan = self.plan
+        th = self.theta.detach().requires_grad_(True)
+        ujet = FBPINNJets.apply(th, plan)
+        params, pleaf = self._params_for_step()
+        loss = self.epilogue.loss(ujet, params)
+        leaves = [th] + ([pleaf] if pleaf is not None else [])
+        grads = torch.autograd.grad(loss, leaves, allow_unused=True)
+        g_p = None
+        if pleaf is not None:
+            g_p = grads[1] if grads[1] is not None else torch.zeros_like(self.pflat)
+        return loss.detach(), grads[0], g_p
+
     def _step_body(self):
         plan = self.plan
         th = self.theta.detach().requires_grad_(True)

@@ -50,9 +50,11 @@ class _Trainer:
     accelerated path): output directories, constants files, summaries and
     model pickles."""
 
-    def __init__(self, c):
-        c.get_outdirs()
-        c.save_constants_file()
+    def __init__(self, c, write_outputs=True):
+        self.write_outputs = write_outputs
+        if write_outputs:
+            c.get_outdirs()
+            c.save_constants_file()
         logger.info(c)
         self.c = c
 
@@ -61,6 +63,8 @@ class _Trainer:
             i, self.c.n_steps, loss, rate, (time.time() - start) / (60 * 60), self.c.run))
 
     def _save_model(self, i, model):
+        if not self.write_outputs:
+            return
         with open(self.c.model_out_dir + f"model_{i:08d}.jax", "wb") as f:
             pickle.dump(_to_numpy(model), f)
 
@@ -87,7 +91,7 @@ class FBPINNTrainer(_Trainer):
         all_params["trainable"]["network"] = {"subdomain": _stack_subdomain_params(plist)}
         return key, all_params
 
-    def _common_init(self, key, all_params):
+    def _common_init(self, key, all_params, exact=True):
         """trainers.py:422-477: constraints, jet requests, test points, exact solution."""
         c = self.c
         for k in all_params["trainable"]:
@@ -103,12 +107,14 @@ class FBPINNTrainer(_Trainer):
         constraints = [list(cst[:-1]) for cst in cg]
         logger.info(f"Total number of constraints: {len(constraints)}")
         x_test = c.domain.sample_interior(all_params=all_params, key=None, sampler="grid", batch_shape=c.n_test)
-        logger.info("Computing exact solution..")
-        try:
-            u_exact = c.problem.exact_solution(all_params=all_params, x_batch=x_test, batch_shape=c.n_test)
-        except NotImplementedError:
-            u_exact = None
-        logger.info("Computing done")
+        u_exact = None
+        if exact:
+            logger.info("Computing exact solution..")
+            try:
+                u_exact = c.problem.exact_solution(all_params=all_params, x_batch=x_test, batch_shape=c.n_test)
+            except NotImplementedError:
+                u_exact = None
+            logger.info("Computing done")
         return key, constraints, required_ujss, x_test, u_exact
 
     def _make_engine(self, all_params):
@@ -116,16 +122,22 @@ class FBPINNTrainer(_Trainer):
         return FusedEngine(c.problem, c.decomposition, c.network, all_params, c.optimiser_kwargs,
                            device=c.device, graph=c.graph)
 
-    def train(self):
+    def setup(self, exact=True):
+        """Parameters, constraints, test points and the engine (no steps)."""
         c = self.c
         key = rng.PRNGKey(c.seed)
         np.random.seed(c.seed)
         key, all_params = self._init_params(key)
         scheduler = c.scheduler(all_params=all_params, n_steps=c.n_steps, **c.scheduler_kwargs)
-        key, constraints, required_ujss, x_test, u_exact = self._common_init(key, all_params)
+        key, constraints, required_ujss, x_test, u_exact = self._common_init(key, all_params, exact)
         eng = self._make_engine(all_params)
         self.engine = eng
         eng.set_constraints(constraints, required_ujss)
+        return eng, scheduler, x_test, u_exact
+
+    def train(self):
+        c = self.c
+        eng, scheduler, x_test, u_exact = self.setup()
         x_test_d = x_test.to(eng.device).contiguous()
         u_exact_d = u_exact.to(eng.device) if u_exact is not None else None
 

@@ -1,0 +1,241 @@
+"""GPU parity of the whole train step for the config problems, against the
+float64 oracle on identical float32 inputs:
+  loss            |gpu - oracle| <= 1e-4 |oracle|
+  gradients       per layer tensor max|gpu - oracle| <= 1e-4 max|oracle|
+  problem params  (learnable mu) same bound
+  Adam            one step of the kernel update vs oracle/adam.py (float32), 1e-6
+  graph replay    every replay equals the eager step (bitwise), with eager work
+                  and synchronisation between replays
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from fbpinns_amd import problems as P
+from fbpinns_amd.domains import RectangularDomainND
+from fbpinns_amd.engine import FusedEngine
+from fbpinns_amd.networks import FCN
+from fbpinns_amd.util import rng
+
+import setups
+from oracle import adam as oadam
+from oracle import model as om
+from oracle import problems as opb
+from oracle import takes as ot
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+class NonAffineHO(P.HarmonicOscillator1DHardBC):
+    """Hard-BC oscillator with a constraining function that is not affine in
+    u (exercises the generic nested-jvp epilogue)."""
+
+    @staticmethod
+    def constraining_fn(all_params, x_batch, u):
+        sd = all_params["static"]["problem"]["sd"]
+        return 1 + torch.tanh(x_batch[:, 0:1] / sd) ** 2 * (u + 0.1 * u ** 3)
+
+
+def _oracle_nonaffine(d, w0, sd):
+    req, _, loss_fn = opb.ho_hard(d, w0, sd)
+    return req, (lambda x, u: 1 + torch.tanh(x[:, 0:1] / sd) ** 2 * (u + 0.1 * u ** 3)), loss_fn
+
+
+# name: (problem, kwargs, lo, hi, nsub, overlap, layers, ns, sampler, oracle(), trainable mu?)
+CASES = {
+    "ho_hard_cfg1": (P.HarmonicOscillator1DHardBC, dict(d=2, w0=80, sd=0.1), (0.0,), (1.0,), (15,), None,
+                     [1, 32, 1], ((200,),), "grid", lambda: opb.ho_hard(2, 80, 0.1), False),
+    "ho_soft_two_constraints": (P.HarmonicOscillator1D, dict(d=2, w0=20), (0.0,), (1.0,), (5,), 2.99,
+                                [1, 32, 1], ((60,),), "grid", lambda: opb.ho_soft(2, 20), False),
+    "ho_inverse_mu": (P.HarmonicOscillator1DInverse, dict(d=2, w0=20), (0.0,), (1.0,), (5,), 2.99, [1, 32, 1],
+                      ((60,),), "grid", lambda: opb.ho_inverse(400), True),
+    "ho_nonaffine_generic": (NonAffineHO, dict(d=2, w0=20, sd=0.1), (0.0,), (1.0,), (6,), 2.5, [1, 32, 32, 1],
+                             ((80,),), "grid", lambda: _oracle_nonaffine(2, 20, 0.1), False),
+    "poisson_small": (P.PoissonMultiscale2D, dict(), (0.0, 0.0), (1.0, 1.0), (4, 4), 1.9, [2, 32, 32, 1],
+                      ((48, 48),), "uniform", lambda: opb.poisson(0.1), False),
+    "burgers_small": (P.BurgersEquation2D, dict(), (-1.0, 0.0), (1.0, 1.0), (6, 3), 2.9, [2, 32, 32, 1],
+                      ((40, 20),), "grid", lambda: opb.burgers(), False),
+    "wave_small": (P.WaveEquationConstantVelocity3D, dict(c0=1, source=P.wave_sources()),
+                   (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0), (3, 3, 2), 1.9, [3, 64, 64, 1], ((12, 12, 12),),
+                   "uniform", lambda: opb.wave(P.wave_sources()), False),
+}
+
+
+def _build(name, seed=0, graph=False):
+    prob, kw, lo, hi, nsub, ov, ls, ns, sampler, ofac, has_mu = CASES[name]
+    if ov is None:   # cfg 1: 15 centres, width 0.15
+        from fbpinns_amd.decompositions import RectangularDecompositionND
+        dec_cls = RectangularDecompositionND
+        xs = [np.linspace(0, 1, 15)]
+        dec = dec_cls.init_params(xs, [0.15 * np.ones(15)], (0.0, 1.0))[0]
+        key = rng.PRNGKey(seed)
+        _, *sub = rng.split(key, dec["m"] + 1)
+        from fbpinns_amd.trainers import _stack_subdomain_params
+        net = _stack_subdomain_params([FCN.init_params(k, ls) for k in sub])
+        ap = {"static": {"decomposition": dec}, "trainable": {"network": {"subdomain": net}}}
+    else:
+        dec_cls, ap = setups.build(nsub, lo, hi, ov, ls, seed=seed)
+    ap["static"]["domain"] = RectangularDomainND.init_params(np.array(lo), np.array(hi))[0]
+    ps = prob.init_params(**kw)
+    ap["static"]["problem"] = ps[0]
+    if ps[1]:
+        ap["trainable"]["problem"] = {k: v.clone() + 0.7 for k, v in ps[1].items()}  # nonzero mu
+    cg = prob.sample_constraints(ap, RectangularDomainND, rng.PRNGKey(seed + 11), sampler, ns)
+    constraints = [list(c[:-1]) for c in cg]
+    req = [c[-1] for c in cg]
+    eng = FusedEngine(prob, dec_cls, FCN, ap, optimiser_kwargs=dict(learning_rate=1e-3), device="cuda:0",
+                      graph=graph)
+    eng.set_constraints(constraints, req)
+    eng.set_active(np.ones(eng.m, dtype=int))
+    return eng, ap, constraints, ofac, has_mu
+
+
+def _oracle_loss_grad(eng, constraints, ofac, has_mu):
+    ap = eng.all_params
+    req, C, L = ofac()
+    p = ap["static"]["decomposition"]["subdomain"]["params"]
+    xmins, xmaxs = p[0].cpu().numpy(), p[1].cpu().numpy()
+    pou = ap["static"]["decomposition"]["subdomain"]["pou"].cpu().numpy().reshape(-1)
+    m = xmins.shape[0]
+    xg = torch.cat([c[0] for c in constraints]).numpy()
+    sizes = [c[0].shape[0] for c in constraints]
+    offs = list(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int))
+    tips, local, cfs, cips = ot.get_x_batch(np.ones(m, dtype=int), xg, offs, sizes, xmins, xmaxs)
+    x = xg[tips]
+    takes, all_ims, *_ = ot.get_inputs(x, np.ones(m, dtype=int), xmins, xmaxs, pou)
+    takess = ot.split_takes(takes, cfs, cips, len(tips))
+    cons = [[torch.tensor(c_.numpy()[loc], dtype=torch.float64) for c_ in c] for c, loc in zip(constraints, local)]
+    layers = [(W.detach().cpu()[all_ims].double(), b.detach().cpu()[all_ims].double())
+              for W, b in ap["trainable"]["network"]["subdomain"]["layers"]]
+    dom = om.to_torch_dom(xmins[all_ims], xmaxs[all_ims], p[4].cpu().numpy()[all_ims], p[5].cpu().numpy()[all_ims],
+                          torch.float64)
+    jmapss = [om.get_jmaps(r) for r in req]
+    extra = ()
+    if has_mu:
+        extra = (eng.pflat[:1].detach().cpu().double().reshape(()),)
+    loss, gl, gx = om.value_and_grad(layers, dom, cons, takess, jmapss, C, L, extra=extra)
+    return loss, gl, gx, all_ims
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_loss_and_gradients(name):
+    eng, ap, constraints, ofac, has_mu = _build(name)
+    print(f"[step] {name} pairs={eng.plan.wl.n_pairs} J={eng.net.J} affine={eng.epilogue.affine is not None}",
+          flush=True)
+    loss, g, gp = eng.loss_and_grad()
+    torch.cuda.synchronize()
+    oloss, ogl, ogx, all_ims = _oracle_loss_grad(eng, constraints, ofac, has_mu)
+    el = abs(loss.item() - oloss.item()) / abs(oloss.item())
+    print(f"  loss gpu {loss.item():.8e} oracle {oloss.item():.8e} rel {el:.2e}")
+    assert el <= TOL
+    g = g.detach().cpu()
+    off = 0
+    for (gW, gb) in ogl:
+        o, i = gW.shape[1], gW.shape[2]
+        kW = g[all_ims, off:off + o * i].reshape(len(all_ims), o, i)
+        off += o * i
+        kb = g[all_ims, off:off + o]
+        off += o
+        for a, b, nm in ((kW, gW, "W"), (kb, gb, "b")):
+            e = ((a.double() - b).abs().max() / b.abs().max().clamp_min(1e-30)).item()
+            print(f"  d{nm}{tuple(b.shape)} rel {e:.2e}")
+            assert e <= TOL, (nm, e)
+    if has_mu:
+        e = abs(gp[0].item() - ogx[0].item()) / abs(ogx[0].item())
+        print(f"  dmu gpu {gp[0].item():.6e} oracle {ogx[0].item():.6e} rel {e:.2e}")
+        assert e <= TOL
+    assert (eng.plan.wl.n_points == sum(c[0].shape[0] for c in constraints))
+
+
+def test_adam_matches_oracle():
+    eng, ap, constraints, ofac, has_mu = _build("ho_inverse_mu")
+    th0, p0 = eng.theta.clone(), eng.pflat.clone()
+    loss, g, gp = eng.loss_and_grad()
+    for k in range(3):
+        eng.step()
+    torch.cuda.synchronize()
+    # oracle: three steps of numpy Adam (float32) on the same gradients would
+    # need the oracle loss at each step; check the first step exactly instead
+    eng2, *_ = _build("ho_inverse_mu")
+    loss, g, gp = eng2.loss_and_grad()
+    eng2.step()
+    rows = eng2.plan.rows.cpu().numpy()
+    pn, mn, vn, c = oadam.adam_update(th0.cpu().numpy()[rows], np.zeros_like(th0.cpu().numpy()[rows]),
+                                      np.zeros_like(th0.cpu().numpy()[rows]), g.cpu().numpy()[rows], 0, lr=1e-3)
+    e = np.abs(eng2.theta.cpu().numpy()[rows] - pn).max()
+    print(f"[adam] max |theta - oracle adam| {e:.3e}")
+    assert e <= 1e-6
+    assert np.allclose(eng2.mu.cpu().numpy()[rows], mn, rtol=1e-6, atol=1e-12)
+    assert np.allclose(eng2.nu.cpu().numpy()[rows], vn, rtol=1e-6, atol=1e-18)
+    pq, *_ = oadam.adam_update(p0.cpu().numpy(), np.zeros(1, np.float32), np.zeros(1, np.float32),
+                               gp.cpu().numpy(), 0, lr=1e-3)
+    assert abs(eng2.pflat.cpu().numpy()[0] - pq[0]) <= 1e-6
+    assert int(eng2.count.item()) == 1 and int(eng.count.item()) == 3
+
+
+@pytest.mark.parametrize("name", ["ho_inverse_mu", "wave_small", "burgers_small"])
+def test_graph_replay_equals_eager(name):
+    ea, *_ = _build(name, graph=False)
+    eb, *_ = _build(name, graph=True)
+    assert eb._graph is not None
+    junk = torch.randn(1 << 20, device="cuda")
+    for k in range(4):
+        la = ea.step()
+        lb = eb.step()
+        torch.cuda.synchronize()
+        s = junk.sum().item()   # eager work between replays
+        assert math.isfinite(s)
+        print(f"[graph] {name} step {k} eager {la.item():.8e} graph {lb.item():.8e}", flush=True)
+        assert la.item() == lb.item()
+        assert torch.equal(ea.theta, eb.theta)
+        assert torch.equal(ea.pflat, eb.pflat)
+        assert int(ea.count.item()) == int(eb.count.item()) == k + 1
+
+
+def test_scheduler_masks_fixed_rows():
+    eng, ap, constraints, ofac, has_mu = _build("burgers_small")
+    m = eng.m
+    active = np.zeros(m, dtype=int)
+    xm = np.array(ap["static"]["decomposition"]["xmins0"])
+    t_layer = np.argsort(np.argsort(xm[:, 1]))  # rough ordering by time
+    active[xm[:, 1] <= np.sort(np.unique(xm[:, 1]))[0]] = 2
+    active[(xm[:, 1] > np.sort(np.unique(xm[:, 1]))[0]) & (xm[:, 1] <= np.sort(np.unique(xm[:, 1]))[1])] = 1
+    print(f"[sched] m={m} active={int((active == 1).sum())} fixed={int((active == 2).sum())}", flush=True)
+    eng.set_active(active)
+    th0 = eng.theta.clone()
+    for _ in range(3):
+        eng.step()
+    torch.cuda.synchronize()
+    fixed = eng.active == 2
+    trained = eng.active == 1
+    unused = eng.active == 0
+    assert torch.equal(eng.theta[torch.as_tensor(fixed)], th0[torch.as_tensor(fixed)])
+    assert torch.equal(eng.theta[torch.as_tensor(unused)], th0[torch.as_tensor(unused)])
+    assert (eng.theta[torch.as_tensor(trained)] != th0[torch.as_tensor(trained)]).any(1).all()
+    assert (eng.mu[torch.as_tensor(fixed)] == 0).all()
+    assert int(eng.count.item()) == 3
+
+
+@pytest.mark.parametrize("name,chunks", [("ho_hard_cfg1", 1), ("ho_inverse_mu", 1), ("wave_small", 3)])
+def test_plain_form_matches_fused(name, chunks):
+    """The plain per-op baseline computes the same step as the fused engine."""
+    from fbpinns_amd.plain import PlainFBPINN
+    eng, ap, constraints, ofac, has_mu = _build(name)
+    prob = CASES[name][0]
+    from fbpinns_amd.decompositions import RectangularDecompositionND
+    pl = PlainFBPINN(prob, RectangularDecompositionND, FCN, eng, point_chunks=chunks)
+    pl.set_active(np.ones(eng.m, dtype=int))
+    for k in range(3):
+        lf = eng.step().item()
+        lp = pl.step().item()
+        e = abs(lf - lp) / abs(lp)
+        print(f"[plain] {name} step {k}: fused {lf:.8e} plain {lp:.8e} rel {e:.2e}", flush=True)
+        assert e <= TOL
+    W = pl.layers[0][0]
+    Wf = eng.all_params["trainable"]["network"]["subdomain"]["layers"][0][0]
+    assert (W - Wf).abs().max().item() <= 1e-5
