@@ -169,8 +169,9 @@ class Epilogue:
             z = torch.zeros((xb.shape[0], 1), dtype=xb.dtype, device=xb.device)
             return C(all_params, xb, z + 1) - C(all_params, xb, z), ()
 
-        Aj = torch.cat(get_ujs(x64, jm, fA), 1).T.contiguous().float()   # (J, n) in req order
-        Bj = torch.cat(get_ujs(x64, jm, fB), 1).T.contiguous().float()
+        dt = self.x.dtype
+        Aj = torch.cat(get_ujs(x64, jm, fA), 1).T.contiguous().to(dt)   # (J, n) in req order
+        Bj = torch.cat(get_ujs(x64, jm, fB), 1).T.contiguous().to(dt)
         # req order equals the jet column order (value, then per axis d1/d2 interleaved) -> remap
         order = []
         for iu, ixs in req:

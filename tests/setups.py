@@ -14,6 +14,10 @@ from oracle import takes as ot
 
 
 def decomposition_params(nsub, lo, hi, overlap, multilevel=False, unnorm=(0.0, 1.0)):
+    if all(n == 1 for n in nsub):   # one subdomain covering [lo, hi] scaled by overlap
+        xs = [np.array([(a + b) / 2]) for a, b in zip(lo, hi)]
+        ws = [np.array([overlap * (b - a)]) for a, b in zip(lo, hi)]
+        return RectangularDecompositionND, RectangularDecompositionND.init_params(xs, ws, unnorm)[0]
     xs = [np.linspace(a, b, n) for a, b, n in zip(lo, hi, nsub)]
     ws = get_subdomain_ws(xs, overlap)
     if multilevel:

@@ -1,0 +1,253 @@
+"""CPU checks of the product's host logic against the oracle, the C-ABI
+library's exports, the loss epilogue's two paths, and the API surface
+(schedulers, constants, domains, decompositions)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import setups
+from fbpinns_amd import _lib, epilogue, takes as ptakes
+from fbpinns_amd.constants import Constants, get_subdomain_ws
+from fbpinns_amd.decompositions import RectangularDecompositionND
+from fbpinns_amd.domains import RectangularDomainND
+from fbpinns_amd.engine import pick_schema
+from fbpinns_amd.ops import FbpxError, NetSpec
+from fbpinns_amd import problems as P
+from fbpinns_amd import schedulers as S
+from fbpinns_amd.util import rng
+from oracle import model as om
+from oracle import takes as ot
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ------------------------------------------------------------------ takes
+
+@pytest.mark.parametrize("seed,ml", [(0, False), (1, False), (2, True)])
+def test_inputs_from_inside_matches_oracle(seed, ml):
+    r = np.random.default_rng(seed)
+    _, dec = setups.decomposition_params((7, 5), (0.0, 0.0), (1.0, 1.0), 1.9, multilevel=ml)
+    lo, hi = dec["subdomain"]["params"][0].numpy(), dec["subdomain"]["params"][1].numpy()
+    pou = dec["subdomain"]["pou"].numpy().reshape(-1)
+    m = lo.shape[0]
+    x = r.uniform(0, 1, (500, 2)).astype(np.float32)
+    active = r.integers(0, 3, m)
+    n_take, m_take, ims = ot.inside_points(x, lo, hi)
+    got = ptakes.inputs_from_inside(n_take, m_take, ims, active, pou)
+    want = ot.get_inputs(x, active, lo, hi, pou)
+    for a, b in zip(got[0][:4], want[0][:4]):
+        assert np.array_equal(a, b)
+    assert got[0][4] == want[0][4]
+    for a, b in zip(got[1:], want[1:]):
+        assert np.array_equal(a, b)
+
+
+def test_split_takes_matches_oracle():
+    r = np.random.default_rng(3)
+    _, dec = setups.decomposition_params((6,), (0.0,), (1.0,), 2.2)
+    lo, hi = dec["subdomain"]["params"][0].numpy(), dec["subdomain"]["params"][1].numpy()
+    xg = r.uniform(0, 1, (90, 1)).astype(np.float32)
+    offs, sizes = [0, 50, 70], [50, 20, 20]
+    tips, local, cfs, cips = ot.get_x_batch(np.ones(6, int), xg, offs, sizes, lo, hi)
+    takes, *_ = ot.get_inputs(xg[tips], np.ones(6, int), lo, hi, np.zeros(6))
+    a = ptakes.split_takes(takes, cfs, cips, len(tips))
+    b = ot.split_takes(takes, cfs, cips, len(tips))
+    for ta, tb in zip(a, b):
+        for u, v in zip(ta[:4], tb[:4]):
+            assert np.array_equal(u, v)
+
+
+def test_build_worklist_invariants():
+    r = np.random.default_rng(5)
+    _, dec = setups.decomposition_params((9, 4), (0.0, 0.0), (1.0, 1.0), 2.5, multilevel=True)
+    lo, hi = dec["subdomain"]["params"][0].numpy(), dec["subdomain"]["params"][1].numpy()
+    pou = dec["subdomain"]["pou"].numpy().reshape(-1)
+    m = lo.shape[0]
+    x = r.uniform(0, 1, (3000, 2)).astype(np.float32)
+    active = np.ones(m, int)
+    active[::4] = 2
+    takes, all_ims, act, a_ims, f_ims = ot.get_inputs(x, active, lo, hi, pou)
+    plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
+    t = {k: v.numpy() for k, v in plan.wl.t.items()}
+    m_take, n_take, p_take, np_take, npou = takes
+    cs = t["chunk_start"]
+    assert cs[0] == 0 and cs[-1] == len(m_take) and np.all(np.diff(cs) > 0) and np.all(np.diff(cs) <= 64)
+    # every chunk belongs to one subdomain and lists its pairs in ascending point order
+    glob = all_ims[m_take]
+    pair_model = np.repeat(t["chunk_model"], np.diff(cs))
+    order = np.argsort(m_take, kind="stable")
+    assert np.array_equal(pair_model, glob[order])
+    assert np.array_equal(t["pair_point"], n_take[order])
+    # trained chunks come first; row ranges cover exactly the active subdomains' chunks
+    ct = plan.chunk_train.numpy().astype(bool)
+    assert np.all(ct[:ct.sum()]) and not np.any(ct[ct.sum():])
+    rcs = plan.row_chunk_start.numpy()
+    assert np.array_equal(plan.rows.numpy(), a_ims)
+    for k, row in enumerate(a_ims):
+        assert set(t["chunk_model"][rcs[k]:rcs[k + 1]]) == {row}
+    # seg_pairs is a permutation grouping the pairs by (point, pou) segment
+    sp = t["seg_pairs"]
+    assert np.array_equal(np.sort(sp), np.arange(len(m_take)))
+    inv = np.empty(len(m_take), np.int64)
+    inv[order] = np.arange(len(m_take))
+    ss = t["seg_pair_start"]
+    for sgi in r.integers(0, len(np_take), 40):
+        ids = sp[ss[sgi]:ss[sgi + 1]]
+        assert np.all(t["pair_seg"][ids] == sgi)
+    pss = t["point_seg_start"]
+    assert pss[-1] == len(np_take) and np.all(np.diff(pss) >= 1)
+
+
+# ------------------------------------------------------------------ epilogue
+
+def test_get_jmaps_matches_oracle():
+    reqs = [((0, ()), (0, (0,)), (0, (0, 0))), ((0, (0, 0)), (0, (1, 1)), (0, (2, 2))),
+            ((0, ()), (0, (0,)), (0, (1,)), (0, (0, 0))), ((0, ()),), ((0, (1,)), (0, (0,)))]
+    for r in reqs:
+        assert epilogue.get_jmaps(r) == om.get_jmaps(r)
+
+
+def test_schema_masks_and_pick():
+    assert epilogue.schema_masks([((0, ()), (0, (0,)), (0, (1,)), (0, (0, 0)))], 2) == (3, 1)
+    assert epilogue.schema_masks([((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], 3) == (7, 7)
+    with pytest.raises(FbpxError):
+        epilogue.schema_masks([((0, (0, 1)),)], 2)
+    with pytest.raises(FbpxError):
+        epilogue.schema_masks([((0, (0, 0, 0)),)], 1)
+    assert pick_schema(3, 7, 1) == (7, 3)
+    assert pick_schema(2, 1, 0) == (3, 0)
+    assert pick_schema(1, 0, 0) == (0, 0)
+
+
+def _problem_params(prob, kw, xd):
+    ap = {"static": {"problem": prob.init_params(**kw)[0]}, "trainable": {}}
+    tr = prob.init_params(**kw)[1]
+    if tr:
+        ap["trainable"]["problem"] = {k: v.double() for k, v in tr.items()}
+    return ap
+
+
+class _Cubic(P.HarmonicOscillator1DHardBC):
+    @staticmethod
+    def constraining_fn(all_params, x_batch, u):
+        return 1 + torch.tanh(x_batch[:, 0:1] / 0.1) ** 2 * (u + 0.2 * u ** 3)
+
+
+@pytest.mark.parametrize("prob,kw,xd,req", [
+    (P.HarmonicOscillator1DHardBC, dict(d=2, w0=20), 1, [((0, ()), (0, (0,)), (0, (0, 0)))]),
+    (P.BurgersEquation2D, dict(), 2, [((0, ()), (0, (0,)), (0, (1,)), (0, (0, 0)))]),
+    (P.WaveEquationConstantVelocity3D, dict(c0=1.3, source=P.wave_sources()), 3,
+     [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))]),
+    (P.PoissonMultiscale2D, dict(), 2, [((0, (0, 0)), (0, (1, 1)))]),
+    (_Cubic, dict(d=2, w0=20), 1, [((0, ()), (0, (0,)), (0, (0, 0)))]),
+])
+def test_epilogue_affine_equals_generic(prob, kw, xd, req):
+    """The Leibniz (affine) path and the nested-jvp (generic) path give the
+    same u jets; the affine detection picks the right path."""
+    torch.manual_seed(0)
+    n = 64
+    x = torch.rand((n, xd), dtype=torch.float64)
+    d1, d2 = pick_schema(xd, *epilogue.schema_masks(req, xd))
+    net = NetSpec.__new__(NetSpec)
+    net.xd, net.d1mask, net.d2mask = xd, d1, d2
+    net.J = 1 + bin(d1).count("1") + bin(d2).count("1")
+    ap = _problem_params(prob, kw, xd)
+    ep = epilogue.Epilogue(prob, ap, x, [(0, n)], [[x]], req, net)
+    assert (ep.affine is not None) == (prob is not _Cubic)
+    q = torch.randn((net.J, n), dtype=torch.float64)
+    if ep.affine is not None:
+        a = ep._ujs_affine(0, q)
+    else:
+        a = None
+    g = ep._ujs_generic(0, q, ap)
+    if a is not None:
+        for u, v in zip(a, g):
+            assert torch.allclose(u, v, rtol=1e-9, atol=1e-9)
+
+
+# ------------------------------------------------------------------ library ABI
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "fbpx.h")).read()
+    declared = set(re.findall(r"\b(fbpx_[a-z_]+)\s*\(", hdr))
+    assert len(declared) >= 15
+    L = _lib.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert set(_lib.SIGNATURES) == declared
+
+
+def test_library_shape_queries_and_error_paths():
+    L = _lib.lib()
+    assert L.fbpx_version().decode().startswith("fbpx")
+    assert _lib.param_count(3, 64, 2) == 3 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    assert _lib.param_count(1, 32, 1) == 97
+    assert _lib.supported(3, 64, 2, 7, 7) and _lib.supported(1, 32, 1, 1, 1)
+    assert not _lib.supported(3, 128, 2, 7, 7) and not _lib.supported(2, 32, 2, 1, 2)
+    # argument validation returns an error code before touching the GPU
+    rc = L.fbpx_inside_count(None, None, -1, 2, None, None, 3, None, None)
+    assert rc == -1 and "bad sizes" in L.fbpx_last_error().decode()
+    n = _lib.fbpx_net(2, 48, 2, 3, 3)
+    rc = L.fbpx_forward(None, ctypes.byref(n), None, None, None, None, None)
+    assert rc == -2 and "not compiled" in L.fbpx_last_error().decode()
+
+
+def test_engine_refuses_to_run_without_gpu():
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from fbpinns_amd.engine import FusedEngine
+    from fbpinns_amd.networks import FCN
+    dec_cls, ap = setups.build((3,), (0.0,), (1.0,), 2.0, [1, 16, 1])
+    with pytest.raises(FbpxError):
+        FusedEngine(P.Problem, dec_cls, FCN, ap)
+
+
+# ------------------------------------------------------------------ API surface
+
+def test_decomposition_params_match_oracle():
+    xs = [np.linspace(0, 1, 6), np.linspace(-1, 1, 4)]
+    ws = get_subdomain_ws(xs, 1.9)
+    dec = RectangularDecompositionND.init_params(xs, ws, (0.1, 2.0))[0]
+    ref = ot.level_params(xs, ws, (0.1, 2.0))
+    for a, b in zip(dec["subdomain"]["params"], ref[:6]):
+        assert np.array_equal(a.numpy(), b.astype(np.float32))
+    assert dec["m"] == 24 and dec["xd"] == 2
+
+
+def test_schedulers():
+    dec = RectangularDecompositionND.init_params([np.linspace(-1, 1, 5), np.linspace(0, 1, 4)],
+                                                 get_subdomain_ws([np.linspace(-1, 1, 5), np.linspace(0, 1, 4)],
+                                                                  2.0), (0, 1))[0]
+    ap = {"static": {"decomposition": dec}}
+    acts = list(S.AllActiveSchedulerND(ap, 5))
+    assert np.all(acts[0] == 1) and all(a is None for a in acts[1:])
+    sl = list(S.SlabSchedulerRectangularND(ap, 8, axis=1, steps_per_slab=2))
+    assert sl[1] is None and sl[3] is None
+    a0, a1 = sl[0].reshape(5, 4), sl[2].reshape(5, 4)
+    assert np.all(a0[:, 0] == 1) and np.all(a0[:, 1:] == 0)
+    assert np.all(a1[:, 0] == 2) and np.all(a1[:, 1] == 1) and np.all(a1[:, 2:] == 0)
+    ln = S.LineSchedulerRectangularND(ap, 100, point=[0.0], iaxis=0)
+    states = [a for a in ln if a is not None]
+    assert len(states) >= 2 and np.all(np.isin(states[-1], [1, 2]))
+    rmin, rmax = S._SubspacePointSchedulerRectangularND._get_radii(np.array([0.5]), np.array([[0.0], [1.0]]),
+                                                                   np.array([[1.0], [2.0]]))
+    assert np.allclose(rmin, [0.0, 0.5]) and np.allclose(rmax, [0.5, 1.5])
+
+
+def test_constants_guard_and_domain_samplers():
+    c = Constants(n_steps=3)
+    assert c.n_steps == 3 and c["seed"] == 0
+    with pytest.raises(KeyError):
+        Constants(not_a_key=1)
+    ap = {"static": {"domain": RectangularDomainND.init_params(np.array([0.0, -1.0]), np.array([1.0, 1.0]))[0]}}
+    g = RectangularDomainND.sample_interior(ap, None, "grid", (3, 2))
+    assert np.allclose(g.numpy(), [[0, -1], [0, 1], [0.5, -1], [0.5, 1], [1, -1], [1, 1]])
+    u = RectangularDomainND.sample_interior(ap, rng.PRNGKey(1), "uniform", (100, 1))
+    assert u.shape == (100, 2) and (u[:, 0] >= 0).all() and (u[:, 1] >= -1).all() and (u[:, 1] <= 1).all()
+    bd = RectangularDomainND.sample_boundaries(ap, rng.PRNGKey(0), "grid", [(4,), (4,), (3,), (3,)])
+    assert torch.all(bd[0][:, 0] == 0) and torch.all(bd[1][:, 0] == 1) and torch.all(bd[3][:, 1] == 1)
