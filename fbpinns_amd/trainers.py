@@ -38,7 +38,9 @@ def _stack_subdomain_params(plist):
 def _to_numpy(t):
     if isinstance(t, dict):
         return {k: _to_numpy(v) for k, v in t.items()}
-    if isinstance(t, (list, tuple)) and not hasattr(t, "_fields"):
+    if isinstance(t, tuple) and hasattr(t, "_fields"):
+        return type(t)(*[_to_numpy(v) for v in t])
+    if isinstance(t, (list, tuple)):
         return type(t)(_to_numpy(v) for v in t)
     if isinstance(t, torch.Tensor):
         return t.detach().cpu().numpy()
