@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libfbpx.so")
 
 FBPX_DOM_STRIDE = 12
 FBPX_MAX_XD = 3
-FBPX_CHUNK = 256
+FBPX_CHUNK = 512
 
 _c_i32 = ctypes.c_int32
 _c_p = ctypes.c_void_p

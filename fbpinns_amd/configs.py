@@ -102,7 +102,38 @@ def burgers(**kw):
     return c
 
 
-CONFIGS = {"ho": ho, "poisson": poisson, "wave": wave, "burgers": burgers}
+def inverse_batch(n_inst=256, seed=3, **kw):
+    """n_inst independent inverse oscillators in one launch: w0 ~ U(20, 100),
+    true mu ~ U(1, 6), subdomain count per instance ~ U{5..40} (seed 3),
+    widths 2.1 x centre spacing, [1,32,1], 200 grid collocation points and 64
+    noisy (sigma 0.01) observations per instance, learnable mu_i from 0."""
+    from fbpinns_amd.util import rng
+    k1, k2, k3 = rng.split(rng.PRNGKey(seed), 3)
+    w0s = rng.uniform(k1, (n_inst,), 20.0, 100.0)
+    mus = rng.uniform(k2, (n_inst,), 1.0, 6.0)
+    ms = np.floor(rng.uniform(k3, (n_inst,), 5.0, 41.0)).astype(int)
+    spacing = 2.0
+    xss = [[np.linspace(0.0, 1.0, m) + spacing * i] for i, m in enumerate(ms)]
+    wss = [get_subdomain_ws(xs, 2.1) for xs in xss]
+    c = Constants(
+        run="inverse_batch",
+        domain=domains.InstancesDomain1D,
+        domain_init_kwargs=dict(n_inst=n_inst, spacing=spacing),
+        problem=problems.BatchedHarmonicOscillatorInverse,
+        problem_init_kwargs=dict(w0s=w0s, mus_true=mus, n_obs=64, noise=0.01, seed=seed, spacing=spacing),
+        decomposition=decompositions.ConcatenatedRectangularDecompositionND,
+        decomposition_init_kwargs=dict(subdomain_xss=xss, subdomain_wss=wss, unnorm=(0.0, 1.0)),
+        network=networks.FCN,
+        network_init_kwargs=dict(layer_sizes=[1, 32, 1]),
+        ns=((200,),), n_test=(200,), n_steps=20000, sampler="grid",
+        optimiser_kwargs=dict(learning_rate=1e-3), seed=seed,
+        summary_freq=5000, test_freq=5000, model_save_freq=10 ** 9)
+    for k, v in kw.items():
+        c[k] = v
+    return c
+
+
+CONFIGS = {"ho": ho, "poisson": poisson, "wave": wave, "burgers": burgers, "inverse": inverse_batch}
 
 # forward FLOPs per (point, subdomain) pair for a [xd, h x nh, 1] network
 # carrying J jet columns: layer 1 value column only (its tangents are

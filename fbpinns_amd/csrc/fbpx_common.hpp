@@ -71,23 +71,12 @@ __device__ inline void load_dom(const float* dom, int model, Dom& d) {
   d.flag = r[10];
 }
 
-// tanh from the identity tanh(z) = sign(z) (1 - e) / (1 + e), e = exp(-2|z|),
-// with an odd Taylor polynomial for |z| < 0.25 to avoid the cancellation in
-// 1 - e (relative error < 4e-7 on both branches).
+// tanh(z) = sign(z) (1 - e) / (1 + e), e = exp(-2|z|), with the hardware
+// exponential and reciprocal; branch-free.  Absolute error <= 1.2e-7 (the
+// 1 - e cancellation near 0 costs relative, not absolute, accuracy).
 __device__ inline float tanh_f(float z) {
-  float a = fabsf(z);
-  float t;
-  if (a < 0.25f) {
-    float z2 = z * z;
-    // z - z^3/3 + 2z^5/15 - 17z^7/315 + 62z^9/2835
-    float p = fmaf(z2, 62.f / 2835.f, -17.f / 315.f);
-    p = fmaf(z2, p, 2.f / 15.f);
-    p = fmaf(z2, p, -1.f / 3.f);
-    t = fmaf(z * z2, p, z);
-    return t;
-  }
-  float e = __expf(-2.f * a);
-  t = (1.f - e) / (1.f + e);
+  const float e = __expf(-2.f * fabsf(z));
+  const float t = (1.f - e) * __builtin_amdgcn_rcpf(1.f + e);
   return copysignf(t, z);
 }
 
@@ -104,13 +93,13 @@ __device__ inline void window_jets(const float* x, const Dom& d, float* w) {
   float c[XD], c1[XD], c2[XD];
 #pragma unroll
   for (int i = 0; i < XD; ++i) {
-    float mu = (d.xmin[i] + d.xmax[i]) / 2.f;
-    float sd = (d.xmax[i] - d.xmin[i]) / 2.f;
-    float th = kPi * (x[i] - mu) / sd;
+    const float mu = (d.xmin[i] + d.xmax[i]) * 0.5f;
+    const float sd = (d.xmax[i] - d.xmin[i]) * 0.5f;
+    const float k = kPi / sd;
+    const float th = (x[i] - mu) * k;
     float s, co;
     sincosf(th, &s, &co);
-    float hc = (1.f + co) / 2.f;
-    float k = kPi / sd;
+    const float hc = (1.f + co) * 0.5f;
     c[i] = hc * hc;
     c1[i] = -k * (1.f + co) * s * 0.5f;
     c2[i] = k * k * (s * s - co * (1.f + co)) * 0.5f;

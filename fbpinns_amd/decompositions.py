@@ -163,3 +163,21 @@ class MultilevelRectangularDecompositionND(RectangularDecompositionND):
                 ps[i].append(p_)
         ps = [np.concatenate(p) for p in ps]
         return RectangularDecompositionND._static(ps, xd)
+
+
+class ConcatenatedRectangularDecompositionND(RectangularDecompositionND):
+    """Several rectangular grids in one decomposition, all in POU 0, e.g.
+    independent problem instances placed side by side on disjoint intervals
+    so that one work list (and one kernel launch) covers all of them.  Each
+    grid is built exactly like a RectangularDecompositionND level
+    (decompositions.py:135-181)."""
+
+    @staticmethod
+    def init_params(subdomain_xss, subdomain_wss, unnorm):
+        xd = len(subdomain_xss[0])
+        ps = [[] for _ in range(7)]
+        for xs, ws in zip(subdomain_xss, subdomain_wss):
+            for i, p_ in enumerate(RectangularDecompositionND._get_level_params(0, xd, xs, ws, unnorm)):
+                ps[i].append(p_)
+        ps = [np.concatenate(p) for p in ps]
+        return RectangularDecompositionND._static(ps, xd)

@@ -98,3 +98,27 @@ class RectangularDomainND(Domain):
             s = rng.uniform(key, (n, xd))
         x = xmin[None] + (xmax - xmin)[None] * s
         return torch.tensor(x, dtype=torch.float32)
+
+
+class InstancesDomain1D(Domain):
+    """n_inst copies of [0, 1] placed at x = spacing * i (independent problem
+    instances sharing one launch).  Samplers return the points of every
+    instance, instance-major."""
+
+    @staticmethod
+    def init_params(n_inst, spacing=2.0):
+        return {"xd": 1, "n_inst": int(n_inst), "spacing": float(spacing),
+                "xmin": np.array([0.0]), "xmax": np.array([spacing * (n_inst - 1) + 1.0])}, {}
+
+    @staticmethod
+    def sample_interior(all_params, key, sampler, batch_shape):
+        d = all_params["static"]["domain"]
+        loc = RectangularDomainND._rectangle_samplerND(key, sampler, np.array([0.0]), np.array([1.0]),
+                                                       batch_shape).double()
+        off = d["spacing"] * torch.arange(d["n_inst"], dtype=torch.float64)
+        return (off[:, None, None] + loc[None]).reshape(-1, 1).float()
+
+    @staticmethod
+    def instance_of(all_params, x_batch):
+        d = all_params["static"]["domain"]
+        return torch.floor(x_batch[:, 0].double() / d["spacing"] + 1e-9).long()

@@ -319,3 +319,65 @@ def wave_sources(seed=2, sigmas=(0.3, 0.1, 0.03), amp=1.0, lo=-0.5, hi=0.5):
     U(lo, hi)^2 by the seeded key."""
     c = rng.uniform(rng.PRNGKey(seed), (len(sigmas), 2), lo, hi)
     return np.concatenate([c, np.asarray(sigmas)[:, None], amp * np.ones((len(sigmas), 1))], axis=1)
+
+
+class BatchedHarmonicOscillatorInverse(Problem):
+    """n_inst independent inverse damped oscillators u'' + mu_i u' + k_i u = 0,
+    u(0) = 1, u'(0) = 0, each with its own learnable mu_i (init 0) fitted to
+    n_obs noisy observations of its analytic solution, as in
+    HarmonicOscillator1DInverse (problems.py:209-271) with a per-instance
+    coefficient.  Instance i lives on x in [spacing*i, spacing*i + 1]
+    (InstancesDomain1D); the loss is the sum of the instances' losses, so each
+    instance's parameters receive exactly their own gradient."""
+
+    @staticmethod
+    def init_params(w0s, mus_true, n_obs=64, noise=0.01, seed=3, spacing=2.0, data_weight=1e6):
+        w0s = np.asarray(w0s, dtype=np.float64)
+        mus_true = np.asarray(mus_true, dtype=np.float64)
+        n = len(w0s)
+        static = {"dims": (1, 1), "n_inst": n, "spacing": float(spacing), "w0": w0s, "mu_true": mus_true,
+                  "d": mus_true / 2, "k": torch.tensor(w0s ** 2, dtype=torch.float32), "n_obs": int(n_obs),
+                  "noise": float(noise), "seed": int(seed), "data_weight": float(data_weight)}
+        return static, {"mu": torch.zeros(n, dtype=torch.float32)}
+
+    @staticmethod
+    def _exact_local(d, w0, t):
+        w = np.sqrt(w0 ** 2 - d ** 2)
+        phi = np.arctan(-d / w)
+        return np.exp(-d * t) * 2 * (1 / (2 * np.cos(phi))) * np.cos(phi + w * t)
+
+    @staticmethod
+    def sample_constraints(all_params, domain, key, sampler, batch_shapes):
+        p = all_params["static"]["problem"]
+        n, sp = p["n_inst"], p["spacing"]
+        x_phys = domain.sample_interior(all_params, key, sampler, batch_shapes[0])
+        inst_phys = torch.repeat_interleave(torch.arange(n), x_phys.shape[0] // n)
+        t_obs = np.linspace(0.0, 1.0, p["n_obs"])
+        x_obs = (sp * np.arange(n)[:, None] + t_obs[None]).reshape(-1, 1)
+        u = BatchedHarmonicOscillatorInverse._exact_local(p["d"][:, None], p["w0"][:, None], t_obs[None])
+        u = u + p["noise"] * rng.normal(rng.PRNGKey(p["seed"]), u.shape)
+        inst_obs = torch.repeat_interleave(torch.arange(n), p["n_obs"])
+        return [[x_phys, inst_phys, ((0, ()), (0, (0,)), (0, (0, 0)))],
+                [torch.tensor(x_obs, dtype=torch.float32), torch.tensor(u.reshape(-1, 1), dtype=torch.float32),
+                 inst_obs, ((0, ()),)]]
+
+    @staticmethod
+    def loss_fn(all_params, constraints):
+        p = all_params["static"]["problem"]
+        mu = all_params["trainable"]["problem"]["mu"]
+        n = p["n_inst"]
+        _, inst, u, ut, utt = constraints[0]
+        k = p["k"].to(u.device)
+        r = utt + mu[inst][:, None] * ut + k[inst][:, None] * u
+        _, uc, inst_o, u_o = constraints[1]
+        # sum over instances of per-instance means (equal point counts per instance)
+        return n * mean(r ** 2) + p["data_weight"] * n * mean((u_o - uc) ** 2)
+
+    @staticmethod
+    def exact_solution(all_params, x_batch, batch_shape=None):
+        p = all_params["static"]["problem"]
+        x = x_batch.detach().cpu().double().numpy()[:, 0]
+        inst = np.floor(x / p["spacing"] + 1e-9).astype(int)
+        t = x - p["spacing"] * inst
+        u = BatchedHarmonicOscillatorInverse._exact_local(p["d"][inst], p["w0"][inst], t)
+        return torch.tensor(u[:, None], dtype=torch.float32)

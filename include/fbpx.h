@@ -57,7 +57,7 @@ typedef struct {
  * (point, pou) row of the reference's jnp.unique (trainers.py:380-385),
  * ordered by (point, pou).
  */
-#define FBPX_CHUNK 256
+#define FBPX_CHUNK 512
 typedef struct {
   int32_t n_pairs;
   int32_t n_chunks;
