@@ -11,6 +11,7 @@ Prints ONE JSON line.  Default workload: the (2+1)D wave configuration
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,6 +27,34 @@ PEAK_HBM = 8000.0         # GB/s
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` on `config`, as recorded from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/<config>_traffic.md
+    (a line `<kernel> traffic_bytes_per_launch = N`), or None."""
+    path = os.path.join(ROOT, "profiles", f"{config}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(rf"^\s*{re.escape(kernel)}\s+traffic_bytes_per_launch\s*=\s*(\d+)")
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line)
+            if m:
+                return int(m.group(1))
+    return None
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size (reads and writes `mib` MiB each
+    launch): the reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "launches": 5}))
 
 
 def time_kernel(fn, reps):
@@ -149,11 +178,17 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-graph", action="store_true", help="launch the step eagerly (for counter passes)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every run here uses one MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
 
     from fbpinns_amd import configs
     from fbpinns_amd.plain import PlainFBPINN
@@ -162,7 +197,7 @@ def main():
     for h in logger.handlers:
         h.setStream(sys.stderr)   # stdout carries only the JSON line
 
-    c = configs.CONFIGS[a.config]()
+    c = configs.CONFIGS[a.config](graph=not a.no_graph)
     log(f"config {a.config}: building problem")
     tr = FBPINNTrainer(c, write_outputs=False)
     eng, sched, x_test, _ = tr.setup(exact=False)
@@ -190,6 +225,8 @@ def main():
         f"bwd {rl['t_bwd_ms']:.3f} ms ({rl['tflops_bwd']:.1f} TFLOP/s)")
     dom_is_bwd = rl["t_bwd_ms"] >= rl["t_fwd_ms"]
     ach = rl["tflops_bwd"] if dom_is_bwd else rl["tflops_fwd"]
+    dom_kernel = "bwd_kernel" if dom_is_bwd else "fwd_kernel"
+    traffic = measured_traffic(a.config, dom_kernel)
 
     plain = None
     if not a.no_plain:
@@ -237,8 +274,9 @@ def main():
                    "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs},
         "collocation_pts_per_s": sps * n_pts,
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP32_MFMA, "traffic": None,
-                     "kernel": "bwd_kernel" if dom_is_bwd else "fwd_kernel"},
+                     "frac": ach / PEAK_FP32_MFMA, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "kernel": dom_kernel},
         "kernels": rl,
         "plain_gpu": plain,
         "speedup_vs_plain": (sps / plain["steps_per_s"]) if plain else None,
