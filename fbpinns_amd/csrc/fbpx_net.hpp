@@ -52,16 +52,17 @@ __device__ inline void stage_small(const float* th, float* ss) {
 }
 
 // Padded LDS copies of W2: wt[k*(H+4) + o] = W2[o][k] (read as the forward A
-// operand) and wr[o*(H+4) + k] = W2[o][k] (read as the W2^T A operand).  The
-// stride H+4 puts lane groups g = 0, 1 of a half-wave 16 banks apart, so both
-// read patterns are bank-conflict-free.
+// operand) and wr[o*(H+4) + k] = WL[o] W2[o][k] (read as the A operand of
+// (diag(WL) W2)^T; the backward carries G / WL, see bwd_two).  The stride H+4
+// puts lane groups g = 0, 1 of a half-wave 16 banks apart, so both read
+// patterns are bank-conflict-free.
 template <int H>
-__device__ inline void stage_w2(const float* w2, float* wt, float* wr) {
+__device__ inline void stage_w2(const float* w2, const float* wl, float* wt, float* wr) {
   for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
     const int o = i / H, k = i % H;
     const float v = w2[i];
     if (wt) wt[k * (H + 4) + o] = v;
-    if (wr) wr[o * (H + 4) + k] = v;
+    if (wr) wr[o * (H + 4) + k] = v * wl[o];
   }
 }
 
@@ -76,32 +77,50 @@ __device__ inline float dpp(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the 16 lanes of a DPP row (lanes 16k..16k+15); every lane of the
-// row receives the sum.  quad_perm[1,0,3,2], quad_perm[2,3,0,1],
-// row_half_mirror, row_mirror.
-__device__ inline float row_sum16(float v) {
-  v += dpp<0xB1>(v);
-  v += dpp<0x4E>(v);
-  v += dpp<0x141>(v);
-  v += dpp<0x140>(v);
-  return v;
+// One reduce-scatter stage over a DPP row: lanes whose stage bit is set keep
+// the upper half of v[0..N), the others the lower half; each adds the half
+// its partner (DPP control CTRL, a lane with the opposite bit) does not keep.
+template <int N, int CTRL>
+__device__ inline void split_stage(float* v, bool upper) {
+#pragma unroll
+  for (int k = 0; k < N / 2; ++k) {
+    const float lo = v[k], hi = v[k + N / 2];
+    const float send = upper ? lo : hi;
+    const float keep = upper ? hi : lo;
+    v[k] = keep + dpp<CTRL>(send);
+  }
 }
 
 // vals[rb][r] holds this lane's contribution for unit 16rb + 4g + r.  Sums
 // every entry over the 16 pair-lanes of the row and adds the total for unit
 // 16*(pl/4) + 4g + pl%4 into dst (lane pl owns that unit; no two lanes of a
-// wave share a destination).
+// wave share a destination).  Butterfly reduce-scatter: partners
+// row_mirror (pl ^ 15), row_half_mirror (pl ^ 7), quad_perm[3,2,1,0]
+// (pl ^ 3), quad_perm[1,0,3,2] (pl ^ 1) flip bits 3, 2, 1, 0 of pl in turn
+// while sharing the bits above; with fewer than 16 values the top stages
+// are plain additions.  After the last stage lane pl holds the total of
+// value index pl & (V - 1), which is multiplied by post before it is added.
 template <int NRB>
-__device__ inline void reduce_units_to_lds(const float (*vals)[4], float* dst, int pl, int g) {
-  float mine = 0.f;
+__device__ inline void reduce_units_to_lds(const float (*vals)[4], float* dst, int pl, int g, float post = 1.f) {
+  constexpr int V = NRB * 4;
+  float v[V];
 #pragma unroll
   for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float s = row_sum16(vals[rb][r]);
-      if (pl == rb * 4 + r) mine = s;
-    }
-  if (pl < NRB * 4) dst[16 * (pl >> 2) + 4 * g + (pl & 3)] += mine;
+    for (int r = 0; r < 4; ++r) v[4 * rb + r] = vals[rb][r];
+  if constexpr (V <= 8) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] += dpp<0x140>(v[k]);
+  }
+  if constexpr (V <= 4) {
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] += dpp<0x141>(v[k]);
+  }
+  if constexpr (V == 16) split_stage<16, 0x140>(v, (pl >> 3) & 1);
+  if constexpr (V >= 8) split_stage<8, 0x141>(v, (pl >> 2) & 1);
+  split_stage<4, 0x1B>(v, (pl >> 1) & 1);
+  split_stage<2, 0xB1>(v, pl & 1);
+  if (pl < V) dst[16 * (pl >> 2) + 4 * g + (pl & 3)] += v[0] * post;
 }
 
 // Hide the values of a register tile from the optimiser (an empty asm that
@@ -133,7 +152,10 @@ __device__ inline float tanh_d3(float h) {
   return -2.f * (d1 * d1 - 2.f * h * h * d1);
 }
 
-// zt[i * H + u] = W1[u][i] / sd_i for the chunk's subdomain, in LDS.
+// Per-subdomain layer-1 tangent tables in LDS (3 * XD * H floats): with
+// zt_i[u] = W1[u][i] / sd_i,
+//   zt[i*H + u] = zt_i[u],  zt[XD*H + i*H + u] = -2 zt_i[u],
+//   zt[2*XD*H + i*H + u] = -2 zt_i[u]^2.
 template <int XD, int H, class SM>
 __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
   for (int t = threadIdx.x; t < XD * H; t += blockDim.x) {
@@ -142,7 +164,10 @@ __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
 #pragma unroll
     for (int j = 1; j < XD; ++j)
       if (i == j) s = isd[j];
-    zt[t] = ss[SM::W1 + u * XD + i] * s;
+    const float z = ss[SM::W1 + u * XD + i] * s;
+    zt[t] = z;
+    zt[XD * H + t] = -2.f * z;
+    zt[2 * XD * H + t] = -2.f * z * z;
   }
 }
 
@@ -153,12 +178,15 @@ __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
 //   w1  = ss + W1 + 4g*XD      W1[16rb + 4g + r][i]        at [(16rb + r)*XD + i]
 //   vec = ss + 4g              B1/B2/WL[16rb + 4g + r]     at [off + 16rb + r]
 //   zt  = szt + 4g             zt[i][16rb + 4g + r]        at [i*H + 16rb + r]
+//   zm, zq: the -2 zt and -2 zt^2 tables, same indexing
 //   wt  = swt + 4g(H+4) + pl   W2[16ob + pl][16kb + 4g + ks] at [(16kb + ks)(H+4) + 16ob]
 //   wr  = swr + 4g(H+4) + pl   W2[16ob + 4g + ks][16ib + pl] at [(16ob + ks)(H+4) + 16ib]
 struct LaneLds {
   const float* w1;
   const float* vec;
   const float* zt;
+  const float* zm;
+  const float* zq;
   const float* wt;
   const float* wr;
 };
@@ -170,6 +198,8 @@ __device__ inline LaneLds lane_lds(const float* ss, const float* szt, const floa
   v.w1 = ss + SM::W1 + 4 * g * XD;
   v.vec = ss + 4 * g;
   v.zt = szt + 4 * g;
+  v.zm = szt + XD * H + 4 * g;
+  v.zq = szt + 2 * XD * H + 4 * g;
   v.wt = swt + 4 * g * (H + 4) + pl;
   v.wr = swr + 4 * g * (H + 4) + pl;
   return v;
@@ -213,10 +243,9 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
       const float h = h1[kb][ks];
       float b = h;
       if (kind == 1) {
-        b = tanh_d1(h) * v.zt[zrow + 16 * kb + ks];
+        b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];            // d1 zt
       } else if (kind == 2) {
-        const float z = v.zt[zrow + 16 * kb + ks];
-        b = tanh_d2(h) * z * z;
+        b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];      // d2 zt^2 = h d1 (-2 zt^2)
       }
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob) {
@@ -238,7 +267,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   using SM = Small<XD, H, NH>;
   constexpr int J = S::J, NRB = H / 16;
   __shared__ float ss[SM::N];
-  __shared__ float szt[XD * H];
+  __shared__ float szt[3 * XD * H];
   __shared__ float swt[NH == 2 ? H * (H + 4) : 1];
 
   const int chunk = blockIdx.x;
@@ -247,7 +276,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
   stage_small<XD, H, NH>(th, ss);
-  if constexpr (NH == 2) stage_w2<H>(th + L::W2, swt, nullptr);
+  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, nullptr);
   Dom d;
   load_dom(dom, model, d);
   float mu[XD], isd[XD];
@@ -296,13 +325,20 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
     } else {
       f4 Z0[NRB];
       layer2_pass<H, SM>(v, h1, 0, 0, Z0);
-      float h2[NRB][4];
+      // output-layer weights times the layer-2 tanh derivatives, per element:
+      // wd1 = WL d1, wd2 = WL d2, so that u_i = sum WL d1 Z_i and
+      // u_ii = sum WL (d1 Z_ii + d2 Z_i^2) cost one or two FMAs per element
+      float wd1[NRB][4], wd2[NRB][4];
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          h2[ob][r] = tanh_f(Z0[ob][r]);
-          ur[0] = fmaf(v.vec[SM::WL + 16 * ob + r], h2[ob][r], ur[0]);
+          const float wlu = v.vec[SM::WL + 16 * ob + r];
+          const float h = tanh_f(Z0[ob][r]);
+          const float d1 = fmaf(-h, h, 1.f);
+          ur[0] = fmaf(wlu, h, ur[0]);
+          wd1[ob][r] = wlu * d1;
+          wd2[ob][r] = (-2.f * wlu) * (h * d1);
         }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -315,12 +351,12 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const float wlu = v.vec[SM::WL + 16 * ob + r];
-            const float h = h2[ob][r];
-            const float d1 = tanh_d1(h);
             const float zi = Zi[ob][r];
-            ur[S::col1(i)] = fmaf(wlu, d1 * zi, ur[S::col1(i)]);
-            if (S::has2(i)) ur[S::col2(i)] = fmaf(wlu, d1 * Zii[ob][r] + tanh_d2(h) * zi * zi, ur[S::col2(i)]);
+            ur[S::col1(i)] = fmaf(wd1[ob][r], zi, ur[S::col1(i)]);
+            if (S::has2(i)) {
+              ur[S::col2(i)] = fmaf(wd1[ob][r], Zii[ob][r], ur[S::col2(i)]);
+              ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
+            }
           }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -356,6 +392,10 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 //     gz1_0 = d1 GA_0 + sum_i d2 zt_i GA_i + sum_i d3 zt_i^2 GA_ii
 //     gz1_i = d1 GA_i + 2 d2 zt_i GA_ii
 //     dW1[u, k] += gz1_0 xh_k + gz1_k / sd_k,  db1 += gz1_0.
+// The two-layer kernel carries G~ = G / WL (row-wise; no division is
+// performed: G~ is formed without the WL factor) and restores WL where G is
+// consumed: W2^T G = (diag(WL) W2)^T G~ through a row-scaled LDS copy of W2,
+// dW2 = diag(WL) sum_c G~_c a1_c^T and db2 = WL sum G~_0 at the end.
 // Small per-unit gradients are summed over the 16 pairs of a sub-tile with
 // DPP row reductions and accumulated per wave in LDS; at the end of the chunk
 // everything is reduced in a fixed order (deterministic, no atomics).
@@ -416,7 +456,7 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   __shared__ float sacc[L::P];
   __shared__ float ssmall[4 * NS * H];
   __shared__ float sbl[64];
-  __shared__ float szt[XD * H];
+  __shared__ float szt[3 * XD * H];
 
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
@@ -599,7 +639,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   static_assert(H >= 16 && H <= 64, "hidden width 16, 32 or 64");
   static_assert(2 * O::TSZ >= L::P, "the gradient row reuses the transpose buffers");
   __shared__ float ss[SM::N];
-  __shared__ float szt[XD * H];
+  __shared__ float szt[3 * XD * H];
   __shared__ float swt[H * TP];
   __shared__ float swr[H * TP];
   __shared__ __attribute__((aligned(16))) float strans[2 * O::TSZ];
@@ -613,7 +653,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
   stage_small<XD, H, 2>(th, ss);
-  stage_w2<H>(th + L::W2, swt, swr);
+  stage_w2<H>(th + L::W2, th + L::WL, swt, swr);
   for (int i = threadIdx.x; i < 4 * NS * H; i += blockDim.x) ssmall[i] = 0.f;
   if (threadIdx.x < 64) sbl[threadIdx.x] = 0.f;
   Dom d;
@@ -631,6 +671,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int pl = lane & 15, g = lane >> 4;
   const int ucol = 16 * O::col_block(wave) + pl;   // this lane's dW2 column unit
+  const float wl_red = ss[SM::WL + 16 * (pl >> 2) + 4 * g + (pl & 3)];   // WL of the unit a reduction leaves here
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
   float* const stbw = stb + (16 * wave + pl) * 16 + 4 * g;          // store bases
   float* const stgw = stg + (16 * wave + pl) * 16 + 4 * g;
@@ -685,9 +726,8 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         for (int r = 0; r < 4; ++r) {
           const float h = tanh_f(Z0[ob][r]);
           h2[ob][r] = h;
-          const float wlu = v.vec[SM::WL + 16 * ob + r];
           vWL[ob][r] = gur[0] * h;
-          G0[ob][r] = tanh_d1(h) * wlu * gur[0];
+          G0[ob][r] = fmaf(-h, h, 1.f) * gur[0];
         }
     }
 #pragma unroll
@@ -705,6 +745,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       const bool two = S::has2(i);
       const float gur_a = pick<J>(gur, S::col1(i));
       const float gur_b = two ? pick<J>(gur, S::col2(i)) : 0.f;
+      const float gur_b2 = 2.f * gur_b;
       const float isd_i = pick<XD>(isd, i);
       const int zrow = i * H;
       opaque<NRB>(h1);
@@ -718,23 +759,23 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
+            // G / WL (the WL factor is carried by the W2^T copy and by the
+            // final row scaling of dW2 and db2)
             const float h = h2[ob][r];
-            const float d1 = tanh_d1(h);
-            const float d2 = -2.f * h * d1;
-            const float wlu = v.vec[SM::WL + 16 * ob + r];
+            const float d1 = fmaf(-h, h, 1.f);
+            const float d2 = -2.f * (h * d1);
             const float zi = Zi[ob][r];
-            const float gai = wlu * gur_a;
+            const float e = d2 * zi;
             float dwl = fmaf(gur_a, d1 * zi, vWL[ob][r]);
-            float g0 = fmaf(d2 * zi, gai, G0[ob][r]);
-            float gi = d1 * gai;
+            float g0 = fmaf(e, gur_a, G0[ob][r]);
+            float gi = d1 * gur_a;
             if (two) {
-              const float d3 = -2.f * (d1 * d1 + h * d2);
+              const float d3 = d1 * fmaf(-6.f, d1, 4.f);   // -2 (d1^2 + h d2)
               const float zii = Zii[ob][r];
-              const float gaii = wlu * gur_b;
-              dwl = fmaf(gur_b, d1 * zii + d2 * zi * zi, dwl);
-              g0 = fmaf(d2 * zii + d3 * zi * zi, gaii, g0);
-              gi = fmaf(2.f * d2 * zi, gaii, gi);
-              Gii[ob][r] = d1 * gaii;
+              dwl = fmaf(gur_b, fmaf(e, zi, d1 * zii), dwl);
+              g0 = fmaf(fmaf(d3 * zi, zi, d2 * zii), gur_b, g0);
+              gi = fmaf(e, gur_b2, gi);
+              Gii[ob][r] = d1 * gur_b;
             }
             vWL[ob][r] = dwl;
             G0[ob][r] = g0;
@@ -754,9 +795,10 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const float h = h1[rb][r];
-            const float z = v.zt[zrow + 16 * rb + r];
-            gzi[rb][r] = tanh_d1(h) * GA[rb][r];
-            gz10[rb][r] = fmaf(tanh_d2(h) * z, GA[rb][r], gz10[rb][r]);
+            const float d1 = fmaf(-h, h, 1.f);
+            const float t = (h * d1) * v.zm[zrow + 16 * rb + r];   // d2 zt
+            gzi[rb][r] = d1 * GA[rb][r];
+            gz10[rb][r] = fmaf(t, GA[rb][r], gz10[rb][r]);
           }
       }
       __syncthreads();
@@ -772,22 +814,20 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const float h = h1[rb][r];
-              const float z = v.zt[zrow + 16 * rb + r];
-              const float d2 = tanh_d2(h);
-              gzi[rb][r] = fmaf(2.f * d2 * z, GA[rb][r], gzi[rb][r]);
-              gz10[rb][r] = fmaf(tanh_d3(h) * z * z, GA[rb][r], gz10[rb][r]);
+              const float d1 = fmaf(-h, h, 1.f);
+              const float t = (h * d1) * v.zm[zrow + 16 * rb + r];   // d2 zt
+              // d3 zt^2 = d1 (3 d1 - 2) (-2 zt^2)
+              const float s3 = (d1 * fmaf(3.f, d1, -2.f)) * v.zq[zrow + 16 * rb + r];
+              gzi[rb][r] = fmaf(t + t, GA[rb][r], gzi[rb][r]);
+              gz10[rb][r] = fmaf(s3, GA[rb][r], gz10[rb][r]);
             }
         }
         __syncthreads();
-        dw2_shared<H, 2>(stgl, stbl, -2.f * zc * zc, dW2, wave);
+        dw2_shared<H, 2>(stgl, stbl, szt[2 * XD * H + zrow + ucol], dW2, wave);
         __syncthreads();
       }
-      // dW1[:, i] through z1_i = zt_i
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) vv[rb][r] = gzi[rb][r] * isd_i;
-      reduce_units_to_lds<NRB>(vv, mysmall + (K_W1 + i) * H, pl, g);
+      // dW1[:, i] through z1_i = zt_i (the 1 / sd_i factor after the sum)
+      reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
     }
     // ---- column 0: db2 += G_0, dW2 += G_0 h^T, GA_0 = W2^T G_0
     opaque<NRB>(h1);
@@ -795,7 +835,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
       for (int r = 0; r < 4; ++r) vv[ob][r] = G0[ob][r];
-    reduce_units_to_lds<NRB>(vv, mysmall + K_B2 * H, pl, g);
+    reduce_units_to_lds<NRB>(vv, mysmall + K_B2 * H, pl, g, wl_red);
     store_tile<H>(stgw, G0);
     {
       f4 GA[NRB];
@@ -803,7 +843,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(tanh_d1(h1[rb][r]), GA[rb][r], gz10[rb][r]);
+        for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(fmaf(-h1[rb][r], h1[rb][r], 1.f), GA[rb][r], gz10[rb][r]);
     }
     __syncthreads();
     dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
@@ -834,7 +874,8 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       for (int j = 0; j < O::NOB; ++j) {
         const int ob = O::row_block(wave, j);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sacc[L::W2 + (16 * ob + 4 * g + r) * H + ucol] += dW2[j][r];
+        for (int r = 0; r < 4; ++r)
+          sacc[L::W2 + (16 * ob + 4 * g + r) * H + ucol] += dW2[j][r] * v.vec[SM::WL + 16 * ob + r];
       }
     }
     __syncthreads();
