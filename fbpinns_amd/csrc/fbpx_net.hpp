@@ -256,6 +256,50 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
   }
 }
 
+// The lane's A-operand fragments of W2 for every layer-2 MFMA, loaded once
+// per chunk: wf[kb][ks][ob] = W2[16ob + pl][16kb + 4g + ks] (H * H / 64
+// registers), so the forward passes read no weights from LDS.
+template <int H>
+__device__ inline void load_w2_frag(const LaneLds& v, float (*wf)[4][H / 16]) {
+#pragma unroll
+  for (int kb = 0; kb < H / 16; ++kb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int ob = 0; ob < H / 16; ++ob) wf[kb][ks][ob] = v.wt[(16 * kb + ks) * (H + 4) + 16 * ob];
+}
+
+// layer2_pass with the weights taken from register fragments.
+template <int H, class SM>
+__device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H / 16], const float (*h1)[4], int zrow,
+                                      int kind, f4* Z) {
+  constexpr int NRB = H / 16;
+#pragma unroll
+  for (int ob = 0; ob < NRB; ++ob) {
+    if (kind == 0) {
+      const float* b2 = v.vec + SM::B2 + 16 * ob;
+      Z[ob] = f4{b2[0], b2[1], b2[2], b2[3]};
+    } else {
+      Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+#pragma unroll
+  for (int kb = 0; kb < NRB; ++kb) {
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const float h = h1[kb][ks];
+      float b = h;
+      if (kind == 1) {
+        b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];
+      } else if (kind == 2) {
+        b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];
+      }
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(wf[kb][ks][ob], b, Z[ob]);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ forward
 
 template <int XD, int H, int NH, int D1M, int D2M>
@@ -292,14 +336,27 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int pl = lane & 15, g = lane >> 4;
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swt, pl, g);
+  float wf[NH == 2 ? NRB : 1][4][NH == 2 ? NRB : 1];
+  if constexpr (NH == 2) load_w2_frag<H>(v, wf);
+  // the point of each sub-tile is fetched one sub-tile ahead (pair index at
+  // the top of the previous sub-tile, coordinates in its middle), so the two
+  // dependent global loads overlap the previous sub-tile's compute
+  float xn[XD];
+  {
+    const int k0 = beg + wave * 16 + pl;
+    const int p0 = wl.pair_point[k0 < end ? k0 : beg];
+#pragma unroll
+    for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)p0 * XD + i];
+  }
   for (int base = beg + wave * 16; base < end; base += nw * 16) {
     const int k = base + pl;
     const bool valid = k < end;
-    const int p = wl.pair_point[valid ? k : beg];
+    const int kn = k + nw * 16;
+    const int pn = wl.pair_point[kn < end ? kn : beg];
     float xv[XD], xh[XD];
 #pragma unroll
     for (int i = 0; i < XD; ++i) {
-      xv[i] = x[(size_t)p * XD + i];
+      xv[i] = xn[i];
       xh[i] = (xv[i] - mu[i]) * isd[i];
     }
     float h1[NRB][4];
@@ -308,6 +365,8 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 #pragma unroll
     for (int c = 0; c < J; ++c) ur[c] = 0.f;
     if constexpr (NH == 1) {
+#pragma unroll
+      for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
@@ -324,7 +383,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
         }
     } else {
       f4 Z0[NRB];
-      layer2_pass<H, SM>(v, h1, 0, 0, Z0);
+      layer2_pass_rf<H, SM>(v, wf, h1, 0, 0, Z0);
       // output-layer weights times the layer-2 tanh derivatives, per element:
       // wd1 = WL d1, wd2 = WL d2, so that u_i = sum WL d1 Z_i and
       // u_ii = sum WL (d1 Z_ii + d2 Z_i^2) cost one or two FMAs per element
@@ -340,13 +399,15 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
           wd1[ob][r] = wlu * d1;
           wd2[ob][r] = (-2.f * wlu) * (h * d1);
         }
+#pragma unroll
+      for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < XD; ++i) {
         if (!S::has1(i)) continue;
         f4 Zi[NRB], Zii[NRB];
-        layer2_pass<H, SM>(v, h1, i * H, 1, Zi);
-        if (S::has2(i)) layer2_pass<H, SM>(v, h1, i * H, 2, Zii);
+        layer2_pass_rf<H, SM>(v, wf, h1, i * H, 1, Zi);
+        if (S::has2(i)) layer2_pass_rf<H, SM>(v, wf, h1, i * H, 2, Zii);
 #pragma unroll
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
@@ -401,14 +462,11 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 // everything is reduced in a fixed order (deterministic, no atomics).
 
 // Upstream gradient of one pair w.r.t. its network output jets:
-// gur = sd_u * (d uw / d u)^T gseg[seg] (zero for padding lanes).
+// gur = sd_u * (d uw / d u)^T gU, gU = gseg[seg] (zero for padding lanes).
 template <class S>
-__device__ inline void pair_upstream(const float* gseg, int seg, bool valid, const float* xv, const Dom& d,
-                                     float* gur) {
+__device__ inline void pair_upstream(const float* gU, const float* xv, const Dom& d, float* gur) {
   constexpr int J = S::J;
-  float gU[J], wj[J];
-#pragma unroll
-  for (int c = 0; c < J; ++c) gU[c] = valid ? gseg[(size_t)seg * J + c] : 0.f;
+  float wj[J];
   window_jets<S>(xv, d, wj);
   jet_mul_T<S>(gU, wj, gur);
 #pragma unroll
@@ -495,8 +553,10 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
       xv[i] = x[(size_t)p * XD + i];
       xh[i] = (xv[i] - mu[i]) * isd[i];
     }
-    float gur[J];
-    pair_upstream<S>(gseg, seg, valid, xv, d, gur);
+    float gur[J], gU[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) gU[c] = valid ? gseg[(size_t)seg * J + c] : 0.f;
+    pair_upstream<S>(gU, xv, d, gur);
     if (g == 0) dbl += gur[0];
     float h1[NRB][4];
     layer1_stream<XD, H, SM>(v, xh, h1);
@@ -698,8 +758,10 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       xv[i] = x[(size_t)p * XD + i];
       xh[i] = (xv[i] - mu[i]) * isd[i];
     }
-    float gur[J];
-    pair_upstream<S>(gseg, seg, valid, xv, d, gur);
+    float gur[J], gU[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) gU[c] = valid ? gseg[(size_t)seg * J + c] : 0.f;
+    pair_upstream<S>(gU, xv, d, gur);
     if (g == 0) dbl += gur[0];
 
     float h1[NRB][4];
