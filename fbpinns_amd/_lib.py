@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libfbpx.so")
 FBPX_DOM_STRIDE = 12
 FBPX_MAX_XD = 3
 FBPX_CHUNK = 512
+FBPX_MAX_REQ = 16
 
 _c_i32 = ctypes.c_int32
 _c_p = ctypes.c_void_p
@@ -54,6 +55,10 @@ SIGNATURES = {
     "fbpx_adam_flat": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_d, _c_d, _c_d, _c_d]),
     "fbpx_count_increment": (_c_i32, [_c_p, _c_p]),
     "fbpx_mean": (_c_i32, [_c_p, _c_p, ctypes.c_int64, _c_p, _c_p]),
+    "fbpx_affine_jets": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_i32, _c_i32, _c_p, _c_p, _c_p, _c_i32,
+                                  ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p)]),
+    "fbpx_affine_jets_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_i32, _c_i32, _c_p, _c_i32,
+                                           ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p), _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_last_error": (ctypes.c_char_p, []),

@@ -216,6 +216,73 @@ __global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, con
   }
 }
 
+// ------------------------------------------------------------------ affine constraining function
+
+struct AffineArgs {
+  int n_req;
+  int kind[FBPX_MAX_REQ];   // 0 value, 1 first, 2 second derivative
+  int ca[FBPX_MAX_REQ];     // column of the first derivative
+  int cb[FBPX_MAX_REQ];     // column of the second derivative
+  float* out[FBPX_MAX_REQ];
+  const float* gout[FBPX_MAX_REQ];
+};
+
+// One thread per point.  Same operation order as the reference's Leibniz
+// expansion evaluated term by term (no contraction into FMAs).
+__global__ void __launch_bounds__(256) affine_jets_kernel(int n, int ld, const float* __restrict__ A,
+                                                          const float* __restrict__ B,
+                                                          const float* __restrict__ q, AffineArgs a) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float A0 = A[k], B0 = B[k], q0 = q[k];
+  for (int r = 0; r < a.n_req; ++r) {
+    float v;
+    if (a.kind[r] == 0) {
+      v = A0 + B0 * q0;
+    } else {
+      const size_t ia = (size_t)a.ca[r] * ld + k;
+      if (a.kind[r] == 1) {
+        v = A[ia] + B[ia] * q0 + B0 * q[ia];
+      } else {
+        const size_t ib = (size_t)a.cb[r] * ld + k;
+        v = A[ib] + B[ib] * q0 + 2.f * B[ia] * q[ia] + B0 * q[ib];
+      }
+    }
+    a.out[r][k] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) affine_jets_backward_kernel(int n, int ld, int J,
+                                                                   const float* __restrict__ B, AffineArgs a,
+                                                                   float* __restrict__ gq) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float g[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) g[c] = 0.f;
+  const float B0 = B[k];
+  for (int r = 0; r < a.n_req; ++r) {
+    const float gr = a.gout[r][k];
+    const int kind = a.kind[r], ca = a.ca[r], cb = a.cb[r];
+    const float Ba = kind >= 1 ? B[(size_t)ca * ld + k] : 0.f;
+    const float Bb = kind == 2 ? B[(size_t)cb * ld + k] : 0.f;
+    // d out / d q0 = B0, Ba or Bb;  d / d qa = B0 (first) or 2 Ba (second);  d / d qb = B0
+    const float d0 = kind == 0 ? B0 : (kind == 1 ? Ba : Bb);
+    const float da = kind == 1 ? B0 : 2.f * Ba;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      float t = c == 0 ? d0 * gr : 0.f;
+      if (kind >= 1 && c == ca) t += da * gr;
+      if (kind == 2 && c == cb) t += B0 * gr;
+      g[c] += t;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 7; ++c)
+    if (c < J) gq[(size_t)c * ld + k] = g[c];
+}
+
 bool schema_supported(int xd, int d1, int d2) {
 #define X(XD, D1, D2) \
   if (xd == XD && d1 == D1 && d2 == D2) return true;
@@ -496,6 +563,80 @@ int fbpx_count_increment(void* stream, int32_t* count) {
   }
   hipLaunchKernelGGL(count_increment_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, count);
   return hip_check(hipGetLastError(), "count_increment");
+}
+
+static int affine_args(const fbpx_net* net, int32_t n, int32_t ld, int32_t n_req, const int32_t* req,
+                       AffineArgs* a, int* J, const char* what) {
+  int rc;
+  if ((rc = check_net_schema(net, J))) return rc;
+  if (n < 0 || ld < n || n_req < 1 || n_req > FBPX_MAX_REQ || !req) {
+    set_error("%s: bad sizes n=%d ld=%d n_req=%d", what, n, ld, n_req);
+    return FBPX_EINVAL;
+  }
+  a->n_req = n_req;
+  for (int r = 0; r < n_req; ++r) {
+    const int c = req[r], xd = net->xd;
+    a->kind[r] = 0;
+    a->ca[r] = a->cb[r] = 0;
+    if (c == 0) continue;
+    const int i = (c - 1) % xd;
+    const bool second = c > xd;
+    if (c < 0 || c > 2 * xd || !((net->d1mask >> i) & 1) || (second && !((net->d2mask >> i) & 1))) {
+      set_error("%s: request code %d not carried by the schema (d1mask=%d d2mask=%d)", what, c, net->d1mask,
+                net->d2mask);
+      return FBPX_EINVAL;
+    }
+    a->kind[r] = second ? 2 : 1;
+    a->ca[r] = 1 + popc(net->d1mask & ((1 << i) - 1));
+    if (second) a->cb[r] = 1 + popc(net->d1mask) + popc(net->d2mask & ((1 << i) - 1));
+  }
+  return FBPX_OK;
+}
+
+int fbpx_affine_jets(void* stream, const fbpx_net* net, int32_t n, int32_t ld, const float* A, const float* B,
+                     const float* q, int32_t n_req, const int32_t* req, float* const* out) {
+  AffineArgs a;
+  int J, rc;
+  if ((rc = affine_args(net, n, ld, n_req, req, &a, &J, "affine_jets"))) return rc;
+  if (n == 0) return FBPX_OK;
+  if (!A || !B || !q || !out) {
+    set_error("affine_jets: null pointer");
+    return FBPX_EINVAL;
+  }
+  for (int r = 0; r < n_req; ++r) {
+    if (!out[r]) {
+      set_error("affine_jets: null output %d", r);
+      return FBPX_EINVAL;
+    }
+    a.out[r] = out[r];
+    a.gout[r] = nullptr;
+  }
+  hipLaunchKernelGGL(affine_jets_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ld, A, B,
+                     q, a);
+  return hip_check(hipGetLastError(), "affine_jets");
+}
+
+int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int32_t ld, const float* B,
+                              int32_t n_req, const int32_t* req, const float* const* gout, float* gq) {
+  AffineArgs a;
+  int J, rc;
+  if ((rc = affine_args(net, n, ld, n_req, req, &a, &J, "affine_jets_backward"))) return rc;
+  if (n == 0) return FBPX_OK;
+  if (!B || !gout || !gq) {
+    set_error("affine_jets_backward: null pointer");
+    return FBPX_EINVAL;
+  }
+  for (int r = 0; r < n_req; ++r) {
+    if (!gout[r]) {
+      set_error("affine_jets_backward: null gradient %d", r);
+      return FBPX_EINVAL;
+    }
+    a.gout[r] = gout[r];
+    a.out[r] = nullptr;
+  }
+  hipLaunchKernelGGL(affine_jets_backward_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                     ld, J, B, a, gq);
+  return hip_check(hipGetLastError(), "affine_jets_backward");
 }
 
 int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out) {

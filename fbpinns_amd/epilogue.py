@@ -10,7 +10,8 @@ kernels, and only the cheap constraining function is differentiated:
 * affine path (constraining_fn(x, u) = A(x) + B(x) u with A, B independent of
   trainable parameters, detected numerically at set-up): the jets of A and B
   are computed once, in float64, with the same nested jvp, and each step
-  applies the Leibniz rule;
+  applies the Leibniz rule in one kernel per constraint (fbpx_affine_jets and
+  its transpose, ops.AffineJets);
 * generic path (anything else): u~ is replaced near each point by its exact
   second-order Taylor polynomial in the requested axes, and the reference's
   nested jvp tree is evaluated on constraining_fn(x, poly(x)), which gives
@@ -23,7 +24,7 @@ calls it (trainers.py:258-267).
 import torch
 from torch.func import jvp
 
-from fbpinns_amd.ops import FbpxError
+from fbpinns_amd.ops import AffineJets, FbpxError
 
 
 # ------------------------------------------------------------------ derivative tree
@@ -185,25 +186,6 @@ class Epilogue:
 
     # -------------------------------------------------------------- per step
 
-    def _ujs_affine(self, ic, q):
-        A, B = self.affine
-        off, n_c = self.cranges[ic]
-        A = A[:, off:off + n_c]
-        B = B[:, off:off + n_c]
-        out = []
-        for iu, ixs in self.required_ujss[ic]:
-            if len(ixs) == 0:
-                v = A[0] + B[0] * q[0]
-            elif len(ixs) == 1:
-                a = self.col[("d1", ixs[0])]
-                v = A[a] + B[a] * q[0] + B[0] * q[a]
-            else:
-                a = self.col[("d1", ixs[0])]
-                b = self.col[("d2", ixs[0])]
-                v = A[b] + B[b] * q[0] + 2 * B[a] * q[a] + B[0] * q[b]
-            out.append(v[:, None])
-        return out
-
     def _ujs_generic(self, ic, q, all_params):
         off, n_c = self.cranges[ic]
         x0 = self.x[off:off + n_c]
@@ -221,16 +203,28 @@ class Epilogue:
             return C(all_params, xb, ut[:, None]), ()
         return get_ujs(x0, self.jmapss[ic], u_fn)
 
+    def _req_codes(self, req):
+        """fbpx_affine_jets request codes of one constraint's required_ujs."""
+        xd = self.net.xd
+        return tuple(0 if len(ixs) == 0 else (1 + ixs[0] if len(ixs) == 1 else 1 + xd + ixs[0])
+                     for iu, ixs in req)
+
     def loss(self, ujet, all_params):
         """ujet: (J, n) jets of u~ at self.x.  Returns the scalar loss."""
         constraints_ = []
-        for ic, (off, n_c) in enumerate(self.cranges):
-            q = ujet[:, off:off + n_c]
-            if self.affine is not None:
-                ujs = self._ujs_affine(ic, q)
-            else:
-                ujs = self._ujs_generic(ic, q, all_params)
-            constraints_.append(list(self.cvalues[ic]) + ujs)
+        if self.affine is not None:
+            # Leibniz rule on the jets in one kernel per constraint
+            A, B = self.affine
+            reqs = [self._req_codes(r) for r in self.required_ujss]
+            outs = AffineJets.apply(ujet, A, B, self.net.as_list(), list(self.cranges), reqs)
+            i = 0
+            for ic, req in enumerate(self.required_ujss):
+                constraints_.append(list(self.cvalues[ic]) + list(outs[i:i + len(req)]))
+                i += len(req)
+        else:
+            for ic, (off, n_c) in enumerate(self.cranges):
+                ujs = self._ujs_generic(ic, ujet[:, off:off + n_c], all_params)
+                constraints_.append(list(self.cvalues[ic]) + ujs)
         return self.problem.loss_fn(all_params, constraints_)
 
     def u(self, ujet, all_params):

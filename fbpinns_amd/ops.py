@@ -18,7 +18,7 @@ import torch
 from fbpinns_amd import _lib
 from fbpinns_amd._lib import FBPX_CHUNK, FBPX_DOM_STRIDE, FbpxError, check, lib
 
-__all__ = ["FbpxError", "NetSpec", "Worklist", "FBPINNJets", "FBPX_CHUNK", "FBPX_DOM_STRIDE"]
+__all__ = ["FbpxError", "NetSpec", "Worklist", "FBPINNJets", "AffineJets", "FBPX_CHUNK", "FBPX_DOM_STRIDE"]
 
 
 def _stream():
@@ -231,6 +231,80 @@ def count_increment(count: torch.Tensor) -> None:
 def mean_into(x: torch.Tensor, scratch: torch.Tensor, out: torch.Tensor) -> None:
     check(lib().fbpx_mean(_stream(), _p(x, torch.float32, "x"), x.numel(), _p(scratch, torch.float32, "scratch"),
                           _p(out, torch.float32, "out")), "fbpx_mean")
+
+
+def _jet_ptr(t, off, name):
+    """Pointer to column 0, point `off` of a contiguous float32 [J][ld] jet array."""
+    return ctypes.c_void_p(_p(t, torch.float32, name).value + 4 * int(off))
+
+
+def _req_arrays(req, tensors, n, name):
+    if not 1 <= len(req) <= _lib.FBPX_MAX_REQ or len(tensors) != len(req):
+        raise FbpxError(f"{name}: {len(req)} requests ({len(tensors)} tensors), at most {_lib.FBPX_MAX_REQ}")
+    for t in tensors:
+        _p(t, torch.float32, name)
+        if t.numel() < n:
+            raise FbpxError(f"{name}: per-request tensor holds {t.numel()} < {n} values")
+    codes = (ctypes.c_int32 * len(req))(*[int(c) for c in req])
+    ptrs = (ctypes.c_void_p * len(req))(*[t.data_ptr() for t in tensors])
+    return codes, ptrs
+
+
+@torch.library.custom_op("fbpx::affine_jets", mutates_args=("outs",))
+def affine_jets(net: List[int], off: int, n: int, A: torch.Tensor, B: torch.Tensor, q: torch.Tensor,
+                req: List[int], outs: List[torch.Tensor]) -> None:
+    ld = q.shape[1]
+    if A.shape != q.shape or B.shape != q.shape or off < 0 or off + n > ld:
+        raise FbpxError(f"affine_jets: jet arrays {tuple(A.shape)}, {tuple(B.shape)}, {tuple(q.shape)}, "
+                        f"range [{off}, {off + n})")
+    codes, ptrs = _req_arrays(req, outs, n, "affine_jets")
+    nt = _net_from_list(net)
+    check(lib().fbpx_affine_jets(_stream(), ctypes.byref(nt), n, ld, _jet_ptr(A, off, "A"), _jet_ptr(B, off, "B"),
+                                 _jet_ptr(q, off, "q"), len(req), codes, ptrs), "fbpx_affine_jets")
+
+
+@torch.library.custom_op("fbpx::affine_jets_backward", mutates_args=("gq",))
+def affine_jets_backward(net: List[int], off: int, n: int, B: torch.Tensor, req: List[int],
+                         gouts: List[torch.Tensor], gq: torch.Tensor) -> None:
+    ld = gq.shape[1]
+    if B.shape != gq.shape or off < 0 or off + n > ld:
+        raise FbpxError(f"affine_jets_backward: jet arrays {tuple(B.shape)}, {tuple(gq.shape)}, "
+                        f"range [{off}, {off + n})")
+    codes, ptrs = _req_arrays(req, gouts, n, "affine_jets_backward")
+    nt = _net_from_list(net)
+    check(lib().fbpx_affine_jets_backward(_stream(), ctypes.byref(nt), n, ld, _jet_ptr(B, off, "B"), len(req),
+                                          codes, ptrs, _jet_ptr(gq, off, "gq")), "fbpx_affine_jets_backward")
+
+
+class AffineJets(torch.autograd.Function):
+    """Requested derivatives of constraining_fn(x, u~) = A + B u~ for every
+    constraint (Leibniz rule on jets, fbpx_affine_jets); one output tensor
+    (n_c, 1) per request, in constraint order.  Gradient w.r.t. the jets of
+    u~ only (A and B do not depend on trainable parameters)."""
+
+    @staticmethod
+    def forward(ctx, ujet, A, B, net, cranges, reqs):
+        outs = []
+        for (off, n_c), req in zip(cranges, reqs):
+            o = [torch.empty((n_c, 1), dtype=torch.float32, device=ujet.device) for _ in req]
+            if n_c:
+                affine_jets(net, off, n_c, A, B, ujet, list(req), o)
+            outs += o
+        ctx.save_for_backward(B)
+        ctx.net, ctx.cranges, ctx.reqs, ctx.shape = net, cranges, reqs, ujet.shape
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        (B,) = ctx.saved_tensors
+        gq = torch.empty(ctx.shape, dtype=torch.float32, device=B.device)
+        i = 0
+        for (off, n_c), req in zip(ctx.cranges, ctx.reqs):
+            g = [t.contiguous() for t in gouts[i:i + len(req)]]
+            i += len(req)
+            if n_c:
+                affine_jets_backward(ctx.net, off, n_c, B, list(req), g, gq)
+        return gq, None, None, None, None, None
 
 
 class _Mean(torch.autograd.Function):

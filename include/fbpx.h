@@ -170,6 +170,37 @@ int fbpx_count_increment(void* stream, int32_t* count);
  * scratch: [256] floats of caller work space. */
 int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out);
 
+/* ---------------------------------------------------------------- constraining function */
+
+/*
+ * Affine constraining function on jets.  When the problem's
+ * constraining_fn(x, u) = A(x) + B(x) u with A, B independent of trainable
+ * parameters (the hard-constraint ansatzes of problems.py:160-175, 289-300,
+ * 359-370), the derivatives that the reference's nested jvp takes of the
+ * constrained solution (trainers.py:174 inside the model differentiated by
+ * _get_ujs/jacfwd, trainers.py:213-247) follow from the jets of A, B and of
+ * the POU-combined output u~ by the Leibniz rule:
+ *   value         A0 + B0 q0
+ *   d/dx_i        Aa + Ba q0 + B0 qa
+ *   d2/dx_i^2     Ab + Bb q0 + 2 Ba qa + B0 qb      (a = col1(i), b = col2(i)).
+ * A, B, q: jets in the fbpx_pou_reduce layout, column c of point k at
+ * c * ld + k (pass the pointers offset to the first point of the constraint).
+ * req: HOST array of n_req <= FBPX_MAX_REQ codes, 0 = value, 1 + i = d/dx_i,
+ * 1 + xd + i = d2/dx_i^2 (the schema must carry it).  out: HOST array of
+ * n_req device pointers, out[r][k] for k < n.
+ */
+#define FBPX_MAX_REQ 16
+int fbpx_affine_jets(void* stream, const fbpx_net* net, int32_t n, int32_t ld, const float* A,
+                     const float* B, const float* q, int32_t n_req, const int32_t* req,
+                     float* const* out);
+
+/* Transpose of fbpx_affine_jets w.r.t. q: gq[c * ld + k] = sum_r
+ * d out_r / d q_c * gout[r][k] for every column c < J of the n points
+ * (columns that no request touches are written as 0). */
+int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int32_t ld,
+                              const float* B, int32_t n_req, const int32_t* req,
+                              const float* const* gout, float* gq);
+
 /* ---------------------------------------------------------------- misc */
 
 /* Parameters per subdomain row for a network shape (0 if unsupported). */

@@ -160,14 +160,24 @@ def test_epilogue_affine_equals_generic(prob, kw, xd, req):
     ep = epilogue.Epilogue(prob, ap, x, [(0, n)], [[x]], req, net)
     assert (ep.affine is not None) == (prob is not _Cubic)
     q = torch.randn((net.J, n), dtype=torch.float64)
-    if ep.affine is not None:
-        a = ep._ujs_affine(0, q)
-    else:
-        a = None
     g = ep._ujs_generic(0, q, ap)
-    if a is not None:
-        for u, v in zip(a, g):
-            assert torch.allclose(u, v, rtol=1e-9, atol=1e-9)
+    if ep.affine is not None:
+        # the precomputed jets of A and B with the Leibniz rule reproduce the
+        # nested-jvp derivatives of constraining_fn (the rule the
+        # fbpx_affine_jets kernel applies; its GPU output is checked against
+        # the same generic path in tests/test_gpu_affine.py)
+        A, B = ep.affine
+        col = epilogue._col_index(net)
+        for (iu, ixs), v in zip(req[0], g):
+            if len(ixs) == 0:
+                u = A[0] + B[0] * q[0]
+            elif len(ixs) == 1:
+                a_ = col[("d1", ixs[0])]
+                u = A[a_] + B[a_] * q[0] + B[0] * q[a_]
+            else:
+                a_, b_ = col[("d1", ixs[0])], col[("d2", ixs[0])]
+                u = A[b_] + B[b_] * q[0] + 2 * B[a_] * q[a_] + B[0] * q[b_]
+            assert torch.allclose(u[:, None], v, rtol=1e-9, atol=1e-9)
 
 
 # ------------------------------------------------------------------ library ABI
@@ -195,6 +205,22 @@ def test_library_shape_queries_and_error_paths():
     n = _lib.fbpx_net(2, 48, 2, 3, 3)
     rc = L.fbpx_forward(None, ctypes.byref(n), None, None, None, None, None)
     assert rc == -2 and "not compiled" in L.fbpx_last_error().decode()
+
+
+def test_affine_jets_validates_requests():
+    L = _lib.lib()
+    n = _lib.fbpx_net(2, 32, 2, 3, 1)   # first derivatives along x0, x1; second along x0 only
+    codes = (ctypes.c_int32 * 1)(4)      # d2/dx1^2: not carried
+    ptrs = (ctypes.c_void_p * 1)(None)
+    rc = L.fbpx_affine_jets(None, ctypes.byref(n), 8, 8, None, None, None, 1, codes, ptrs)
+    assert rc == -1 and "not carried" in L.fbpx_last_error().decode()
+    codes = (ctypes.c_int32 * 1)(0)
+    rc = L.fbpx_affine_jets(None, ctypes.byref(n), 8, 4, None, None, None, 1, codes, ptrs)
+    assert rc == -1 and "bad sizes" in L.fbpx_last_error().decode()
+    rc = L.fbpx_affine_jets_backward(None, ctypes.byref(n), 8, 8, None, 0, codes, ptrs, None)
+    assert rc == -1 and "bad sizes" in L.fbpx_last_error().decode()
+    # n == 0 is a valid empty call
+    assert L.fbpx_affine_jets(None, ctypes.byref(n), 0, 0, None, None, None, 1, codes, ptrs) == 0
 
 
 def test_engine_refuses_to_run_without_gpu():
