@@ -32,7 +32,7 @@ class fbpx_worklist(ctypes.Structure):
     _fields_ = [
         ("n_pairs", _c_i32), ("n_chunks", _c_i32), ("n_points", _c_i32), ("n_segs", _c_i32), ("npou", _c_i32),
         ("pair_point", _c_p), ("pair_seg", _c_p), ("chunk_model", _c_p), ("chunk_start", _c_p),
-        ("point_seg_start", _c_p), ("seg_pair_start", _c_p), ("seg_pairs", _c_p),
+        ("point_seg_start", _c_p), ("seg_pair_start", _c_p), ("pair_slot", _c_p),
     ]
 
 

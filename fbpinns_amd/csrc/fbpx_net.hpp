@@ -351,6 +351,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   for (int base = beg + wave * 16; base < end; base += nw * 16) {
     const int k = base + pl;
     const bool valid = k < end;
+    const int slot = wl.pair_slot[valid ? k : beg];
     const int kn = k + nw * 16;
     const int pn = wl.pair_point[kn < end ? kn : beg];
     float xv[XD], xh[XD];
@@ -434,7 +435,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
     if (valid) {
 #pragma unroll
       for (int c = 0; c < J; ++c)
-        if ((c & 3) == g) pairbuf[(size_t)k * J + c] = uw[c];
+        if ((c & 3) == g) pairbuf[(size_t)slot * J + c] = uw[c];
     }
   }
 }

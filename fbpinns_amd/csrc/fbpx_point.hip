@@ -96,8 +96,9 @@ __global__ void __launch_bounds__(256) window_pairs_kernel(const float* __restri
 #pragma unroll
     for (int i = 0; i < XD; ++i) xv[i] = x[(size_t)p * XD + i];
     window_jets<S>(xv, d, wj);
+    const size_t slot = wl.pair_slot[k];
 #pragma unroll
-    for (int c = 0; c < J; ++c) wpair[(size_t)k * J + c] = wj[c];
+    for (int c = 0; c < J; ++c) wpair[slot * J + c] = wj[c];
   }
 }
 
@@ -109,10 +110,9 @@ __global__ void __launch_bounds__(256) seg_sum_kernel(fbpx_worklist wl, const fl
   float acc[J];
 #pragma unroll
   for (int c = 0; c < J; ++c) acc[c] = 0.f;
-  for (int j = wl.seg_pair_start[seg]; j < wl.seg_pair_start[seg + 1]; ++j) {
-    const size_t id = wl.seg_pairs[j];
+  for (size_t j = wl.seg_pair_start[seg]; j < (size_t)wl.seg_pair_start[seg + 1]; ++j) {
 #pragma unroll
-    for (int c = 0; c < J; ++c) acc[c] += vals[id * J + c];
+    for (int c = 0; c < J; ++c) acc[c] += vals[j * J + c];
   }
 #pragma unroll
   for (int c = 0; c < J; ++c) out[(size_t)seg * J + c] = acc[c];
@@ -178,10 +178,9 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
       U[c] = 0.f;
       W[c] = wsum[(size_t)seg * J + c];
     }
-    for (int j = wl.seg_pair_start[seg]; j < wl.seg_pair_start[seg + 1]; ++j) {
-      const size_t id = wl.seg_pairs[j];
+    for (size_t j = wl.seg_pair_start[seg]; j < (size_t)wl.seg_pair_start[seg + 1]; ++j) {
 #pragma unroll
-      for (int c = 0; c < J; ++c) U[c] += pairbuf[id * J + c];
+      for (int c = 0; c < J; ++c) U[c] += pairbuf[j * J + c];
     }
     jet_div<S>(U, W, q);
 #pragma unroll
@@ -322,6 +321,9 @@ __global__ void __launch_bounds__(256) reduce_rows_kernel(int P, const float* __
   if (idx >= (int64_t)n_rows * P) return;
   const int row = (int)(idx / P), e = (int)(idx % P);
   float g = 0.f;
+  // loads unrolled so several chunk rows are in flight; the sum stays in
+  // ascending chunk order
+#pragma unroll 4
   for (int c = rcs[row]; c < rcs[row + 1]; ++c) g += partial[(size_t)c * P + e];
   grad[(size_t)rows[row] * P + e] = g;
 }
@@ -331,12 +333,18 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
                                                         const float* __restrict__ grad,
                                                         const int32_t* __restrict__ rows, int n_rows,
                                                         const int32_t* __restrict__ count, AdamConst k) {
+  // bias corrections once per workgroup
+  __shared__ float sbc[2];
+  if (threadIdx.x == 0) {
+    const float cf = (float)(count[0] + 1);
+    sbc[0] = 1.f - powf(k.b1, cf);
+    sbc[1] = 1.f - powf(k.b2, cf);
+  }
+  __syncthreads();
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n_rows * P) return;
   const int row = (int)(idx / P), e = (int)(idx % P);
-  const float cf = (float)(count[0] + 1);
-  const float bc1 = 1.f - powf(k.b1, cf);
-  const float bc2 = 1.f - powf(k.b2, cf);
+  const float bc1 = sbc[0], bc2 = sbc[1];
   const size_t i = (size_t)rows[row] * P + e;
   float p = theta[i], m = mu[i], v = nu[i];
   adam_elem(p, m, v, grad[i], bc1, bc2, k);

@@ -74,7 +74,7 @@ def _net_from_list(net: List[int]):
 # ------------------------------------------------------------------ work list
 
 _WL_TENSORS = ["pair_point", "pair_seg", "chunk_model", "chunk_start", "point_seg_start", "seg_pair_start",
-               "seg_pairs"]
+               "pair_slot"]
 _WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou"]
 
 

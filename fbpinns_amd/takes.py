@@ -150,8 +150,9 @@ def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK)
     Tile order groups the pairs by position in all_ims = [active; fixed]
     (stable, so ascending point order inside a subdomain); each subdomain's
     run is cut into chunks of <= `chunk` pairs.  Segments are the unique
-    (point, pou) rows; seg_pairs lists each segment's pairs (tile-order ids)
-    in the reference's pair order."""
+    (point, pou) rows; pair_slot gives each tile-order pair its slot in
+    segment order (a segment's pairs are contiguous there, in the reference's
+    pair order)."""
     m_take, n_take, p_take, np_take, npou = takes
     s = len(m_take)
     n_all = len(all_ims)
@@ -172,7 +173,8 @@ def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK)
     tile_pos[order] = np.arange(s)
     n_segs = len(np_take)
     seg_order = np.argsort(p_take, kind="stable")
-    seg_pairs = tile_pos[seg_order].astype(np.int32)
+    pair_slot = np.empty(s, dtype=np.int32)
+    pair_slot[tile_pos[seg_order]] = np.arange(s, dtype=np.int32)
     seg_pair_start = np.concatenate([[0], np.cumsum(np.bincount(p_take, minlength=n_segs))]).astype(np.int32)
     point_seg_start = np.searchsorted(np_take, np.arange(n_points + 1), side="left").astype(np.int32)
 
@@ -184,7 +186,7 @@ def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK)
         "chunk_start": dev(chunk_start),
         "point_seg_start": dev(point_seg_start),
         "seg_pair_start": dev(seg_pair_start),
-        "seg_pairs": dev(seg_pairs),
+        "pair_slot": dev(pair_slot),
     }
     sizes = {"n_pairs": s, "n_chunks": len(chunk_pos), "n_points": n_points, "n_segs": n_segs, "npou": npou}
     wl = Worklist(tensors, sizes)

@@ -69,8 +69,11 @@ typedef struct {
   const int32_t* chunk_model;     /* [n_chunks] subdomain row of the chunk */
   const int32_t* chunk_start;     /* [n_chunks+1] tile-order start, last = n_pairs */
   const int32_t* point_seg_start; /* [n_points+1] segments of each point */
-  const int32_t* seg_pair_start;  /* [n_segs+1] into seg_pairs */
-  const int32_t* seg_pairs;       /* [n_pairs] tile-order pair ids grouped by segment */
+  const int32_t* seg_pair_start;  /* [n_segs+1] slot range of each segment */
+  const int32_t* pair_slot;       /* [n_pairs] tile order -> slot in segment order: the
+                                     pairs of segment s occupy slots
+                                     seg_pair_start[s] .. seg_pair_start[s+1]-1, in the
+                                     reference's pair order */
 } fbpx_worklist;
 
 /* ---------------------------------------------------------------- takes */
@@ -98,7 +101,8 @@ int fbpx_inside_fill(void* stream, const float* x, int32_t n, int32_t xd,
 /* Window jets summed per segment: wsum[seg][J] = sum over the segment's pairs
  * of the jets of window_fn (decompositions.py:196-199, windows.py:25-35).
  * Parameter-independent; computed once per work list.  pair_scratch:
- * [n_pairs][J] caller-provided work space (e.g. the forward pair buffer). */
+ * [n_pairs][J] caller-provided work space (e.g. the forward pair buffer),
+ * written by slot. */
 int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom,
                      const float* x, const fbpx_worklist* wl, float* pair_scratch,
                      float* wsum);
@@ -106,7 +110,8 @@ int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom,
 /* Per-pair jets of u*w, where u = unnorm(FCN(norm(x))) and w = window(x),
  * i.e. the vmapped FBPINN_model_inner (trainers.py:113-118,158) with the
  * nested-jvp tangents of _get_ujs (trainers.py:213-247) carried in registers.
- * theta: [m][P]; pairbuf: [n_pairs][J] (tile order). */
+ * theta: [m][P]; pairbuf: [n_pairs][J], row = the pair's slot (segment
+ * order), so that fbpx_pou_reduce reads each segment's rows contiguously. */
 int fbpx_forward(void* stream, const fbpx_net* net, const float* theta,
                  const float* dom, const float* x, const fbpx_worklist* wl,
                  float* pairbuf);

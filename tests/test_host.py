@@ -90,9 +90,11 @@ def test_build_worklist_invariants():
     assert np.array_equal(plan.rows.numpy(), a_ims)
     for k, row in enumerate(a_ims):
         assert set(t["chunk_model"][rcs[k]:rcs[k + 1]]) == {row}
-    # seg_pairs is a permutation grouping the pairs by (point, pou) segment
-    sp = t["seg_pairs"]
-    assert np.array_equal(np.sort(sp), np.arange(len(m_take)))
+    # pair_slot is a permutation; the slots of a segment's pairs are its range
+    slot = t["pair_slot"]
+    assert np.array_equal(np.sort(slot), np.arange(len(m_take)))
+    sp = np.empty(len(m_take), np.int64)
+    sp[slot] = np.arange(len(m_take))          # slot -> tile-order pair
     inv = np.empty(len(m_take), np.int64)
     inv[order] = np.arange(len(m_take))
     ss = t["seg_pair_start"]
