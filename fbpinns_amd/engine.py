@@ -20,8 +20,8 @@ the whole step is captured once per work list into a hipGraph and replayed.
 import numpy as np
 import torch
 
-from fbpinns_amd import ops
-from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
+from fbpinns_amd import autotune, ops
+from fbpinns_amd._lib import FBPX_CHUNK, FBPX_DOM_STRIDE, FBPX_MAX_XD
 from fbpinns_amd.epilogue import Epilogue, schema_masks
 from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
 from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch
@@ -77,7 +77,7 @@ class FusedEngine:
     """Owns the packed device state and runs train steps for one problem."""
 
     def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
-                 graph=True):
+                 graph=True, autotune=True):
         from fbpinns_amd.decompositions import RectangularDecompositionND
         from fbpinns_amd.networks import FCN
         if not torch.cuda.is_available():
@@ -89,6 +89,7 @@ class FusedEngine:
         self.problem, self.decomposition, self.network = problem, decomposition, network
         self.device = torch.device(device)
         self.graph_enabled = bool(graph)
+        self.autotune_enabled = bool(autotune)
         ok = dict(optimiser_kwargs or {})
         self.lr = float(ok.pop("learning_rate", 1e-3))
         self.b1 = float(ok.pop("b1", 0.9))
@@ -189,9 +190,10 @@ class FusedEngine:
         if not self.net.supported():
             raise FbpxError(f"no kernels for {self.net}")
 
-    def set_active(self, active):
+    def set_active(self, active, chunk=None):
         """New scheduler state: rebuild takes, work list, epilogue and graph
-        (trainers.py:509-575 and the recompile at :647-653)."""
+        (trainers.py:509-575 and the recompile at :647-653).  chunk: pairs
+        per workgroup (default: autotuned, or FBPX_CHUNK with autotune off)."""
         active = np.asarray(active).astype(np.int64)
         training_ips, cfs, cips, dstat = get_x_batch(active, self.all_params, self.x_global, self.c_offsets,
                                                      self.c_sizes, self.decomposition)
@@ -207,7 +209,16 @@ class FusedEngine:
             off += n_c
         takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
         n_active = int((active_ == 1).sum())
-        plan = build_worklist(takes, all_ims, n_active, x.shape[0], self.device)
+
+        def build(chunk):
+            return build_worklist(takes, all_ims, n_active, x.shape[0], self.device, chunk=chunk)
+        if chunk is not None:
+            self.chunk, plan = int(chunk), build(int(chunk))
+        elif self.autotune_enabled:
+            self.chunk, plan = autotune.tune_chunk(build, self.net, self.theta, self.dom, x, self.m,
+                                                   int(len(takes[0])))
+        else:
+            self.chunk, plan = FBPX_CHUNK, build(FBPX_CHUNK)
         wsum = torch.empty((max(plan.wl.n_segs, 1), self.net.J), dtype=torch.float32, device=self.device)
         scratch = torch.empty((max(plan.wl.n_pairs, 1), self.net.J), dtype=torch.float32, device=self.device)
         ops.pou_prepare(self.net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)
@@ -220,7 +231,8 @@ class FusedEngine:
         self.takes = takes
         self.epilogue = Epilogue(self.problem, self.all_params, x, cranges, cvalues, self.required_ujss, self.net)
         self.n_active = n_active
-        logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks, "
+        logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks "
+                    f"of <= {self.chunk}, "
                     f"{n_active} trained / {len(all_ims)} used subdomains, jets J={self.net.J}, "
                     f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}, d={dstat:.2f}")
         self._graph = None

@@ -122,7 +122,8 @@ class FBPINNTrainer(_Trainer):
     def _make_engine(self, all_params):
         c = self.c
         return FusedEngine(c.problem, c.decomposition, c.network, all_params, c.optimiser_kwargs,
-                           device=c.device, graph=c.graph)
+                           device=c.device, graph=c.graph,
+                           autotune=c.autotune)
 
     def setup(self, exact=True):
         """Parameters, constraints, test points and the engine (no steps)."""
@@ -217,7 +218,8 @@ class PINNTrainer(FBPINNTrainer):
     def _make_engine(self, all_params):
         c = self.c
         return FusedEngine(c.problem, RectangularDecompositionND, c.network, all_params, c.optimiser_kwargs,
-                           device=c.device, graph=c.graph)
+                           device=c.device, graph=c.graph,
+                           autotune=c.autotune)
 
     def train(self):
         from fbpinns_amd.schedulers import AllActiveSchedulerND

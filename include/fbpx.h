@@ -52,8 +52,9 @@ typedef struct {
 /*
  * Work list for one set of takes.  Pairs are (point, subdomain) memberships.
  * "Tile order": pairs grouped by subdomain (ascending chunk index), within a
- * subdomain in ascending point order.  Each chunk is a contiguous run of at
- * most FBPX_CHUNK pairs of ONE subdomain.  "Segment": a unique
+ * subdomain in ascending point order.  Each chunk is a contiguous run of
+ * pairs of ONE subdomain, one workgroup each (any length >= 1; FBPX_CHUNK is
+ * the default, the engine's autotuner may pick another per shape).  "Segment": a unique
  * (point, pou) row of the reference's jnp.unique (trainers.py:380-385),
  * ordered by (point, pou).
  */

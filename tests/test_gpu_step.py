@@ -197,6 +197,29 @@ def test_graph_replay_equals_eager(name):
         assert int(ea.count.item()) == int(eb.count.item()) == k + 1
 
 
+def test_chunk_size_changes_only_summation_order(monkeypatch):
+    """Every autotuner candidate chunk size gives the same loss (the pair jets
+    do not depend on the chunking) and gradients equal to float32 summation
+    accuracy; the autotuner returns one of the candidates."""
+    from fbpinns_amd import autotune
+    monkeypatch.setattr(autotune, "MIN_PAIRS", 0)
+    autotune._cache.clear()
+    eng, *_ = _build("wave_small")
+    print(f"[chunk] autotuned chunk {eng.chunk}, pairs {eng.plan.wl.n_pairs}", flush=True)
+    assert eng.chunk in autotune.CANDIDATES
+    ones = np.ones(eng.m, dtype=int)
+    eng.set_active(ones, chunk=512)
+    l0, g0, _ = eng.loss_and_grad()
+    for c in (1, 64, 256, 1024, 2048):
+        eng.set_active(ones, chunk=c)
+        assert eng.plan.wl.n_chunks >= eng.plan.wl.n_pairs // max(c, 1)
+        l1, g1, _ = eng.loss_and_grad()
+        e = ((g1 - g0).abs().max() / g0.abs().max()).item()
+        print(f"  chunk {c}: chunks {eng.plan.wl.n_chunks} loss {l1.item():.8e} grad rel {e:.2e}", flush=True)
+        assert l1.item() == l0.item()
+        assert e < 1e-5
+
+
 def test_scheduler_masks_fixed_rows():
     eng, ap, constraints, ofac, has_mu = _build("burgers_small")
     m = eng.m
