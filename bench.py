@@ -161,10 +161,21 @@ def cpu_baseline(c, eng, budget_s=20.0):
     s_sample = len(takes[0])
     full_pairs = eng.plan.wl.n_pairs
     rate = (s_sample / dt) / full_pairs
-    return {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port",
+    return {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port", "cpu_model": _cpu_model(),
             "sample": f"oracle (torch CPU float32, nested jvp + autograd + Adam) on the first {n} of {n_full} "
                       f"points ({s_sample} of {full_pairs} pairs), {steps} timed steps of {dt:.2f} s; "
                       f"steps/s extrapolated by pair count"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
