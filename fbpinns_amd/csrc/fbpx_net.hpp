@@ -53,7 +53,8 @@ __device__ inline void stage_small(const float* th, float* ss) {
 
 // Padded LDS copies of W2: wt[k*(H+4) + o] = W2[o][k] (read as the forward A
 // operand) and wr[o*(H+4) + k] = WL[o] W2[o][k] (read as the A operand of
-// (diag(WL) W2)^T; the backward carries G / WL, see bwd_two).  The stride H+4
+// (diag(WL) W2)^T; the backward carries G / WL, see bwd_two), or W2[o][k]
+// itself when wl is null.  The stride H+4
 // puts lane groups g = 0, 1 of a half-wave 16 banks apart, so both read
 // patterns are bank-conflict-free.
 template <int H>
@@ -62,7 +63,7 @@ __device__ inline void stage_w2(const float* w2, const float* wl, float* wt, flo
     const int o = i / H, k = i % H;
     const float v = w2[i];
     if (wt) wt[k * (H + 4) + o] = v;
-    if (wr) wr[o * (H + 4) + k] = v * wl[o];
+    if (wr) wr[o * (H + 4) + k] = wl ? v * wl[o] : v;
   }
 }
 
@@ -313,6 +314,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   __shared__ float ss[SM::N];
   __shared__ float szt[3 * XD * H];
   __shared__ float swt[NH == 2 ? H * (H + 4) : 1];
+  __shared__ float swr[NH == 2 && S::N2 > 0 ? H * (H + 4) : 1];
 
   const int chunk = blockIdx.x;
   const int model = wl.chunk_model[chunk];
@@ -320,7 +322,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
   stage_small<XD, H, NH>(th, ss);
-  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, nullptr);
+  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, S::N2 > 0 ? swr : nullptr);
   Dom d;
   load_dom(dom, model, d);
   float mu[XD], isd[XD];
@@ -335,7 +337,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int pl = lane & 15, g = lane >> 4;
-  const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swt, pl, g);
+  const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
   float wf[NH == 2 ? NRB : 1][4][NH == 2 ? NRB : 1];
   if constexpr (NH == 2) load_w2_frag<H>(v, wf);
   // the point of each sub-tile is fetched one sub-tile ahead (pair index at
@@ -386,8 +388,8 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
       f4 Z0[NRB];
       layer2_pass_rf<H, SM>(v, wf, h1, 0, 0, Z0);
       // output-layer weights times the layer-2 tanh derivatives, per element:
-      // wd1 = WL d1, wd2 = WL d2, so that u_i = sum WL d1 Z_i and
-      // u_ii = sum WL (d1 Z_ii + d2 Z_i^2) cost one or two FMAs per element
+      // wd1 = WL d1, wd2 = WL d2, so that u_i = sum WL d1 Z_i and the Z_i^2
+      // part of u_ii = sum WL (d1 Z_ii + d2 Z_i^2) cost one or two FMAs
       float wd1[NRB][4], wd2[NRB][4];
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob)
@@ -403,23 +405,45 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 #pragma unroll
       for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
       __builtin_amdgcn_sched_barrier(0);
+      // sum_o WL d1 Z_ii = sum_u vt[u] a1_ii[u] with vt = W2^T (WL d1): one
+      // W2^T pass per sub-tile replaces the Z_ii passes of every axis
+      f4 vt[NRB];
+      if constexpr (S::N2 > 0) {
+#pragma unroll
+        for (int ib = 0; ib < NRB; ++ib) vt[ib] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+          for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int ib = 0; ib < NRB; ++ib)
+              vt[ib] = mfma16(v.wr[(16 * ob + ks) * (H + 4) + 16 * ib], wd1[ob][ks], vt[ib]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
 #pragma unroll
       for (int i = 0; i < XD; ++i) {
         if (!S::has1(i)) continue;
-        f4 Zi[NRB], Zii[NRB];
+        f4 Zi[NRB];
         layer2_pass_rf<H, SM>(v, wf, h1, i * H, 1, Zi);
-        if (S::has2(i)) layer2_pass_rf<H, SM>(v, wf, h1, i * H, 2, Zii);
 #pragma unroll
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const float zi = Zi[ob][r];
             ur[S::col1(i)] = fmaf(wd1[ob][r], zi, ur[S::col1(i)]);
-            if (S::has2(i)) {
-              ur[S::col2(i)] = fmaf(wd1[ob][r], Zii[ob][r], ur[S::col2(i)]);
-              ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
-            }
+            if (S::has2(i)) ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
           }
+        if (S::has2(i)) {
+          // + sum_u vt[u] a1_ii[u], a1_ii = d2(h1) zt_i^2 = h d1 (-2 zt_i^2)
+#pragma unroll
+          for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float h = h1[rb][r];
+              const float a = (h * fmaf(-h, h, 1.f)) * v.zq[i * H + 16 * rb + r];
+              ur[S::col2(i)] = fmaf(vt[rb][r], a, ur[S::col2(i)]);
+            }
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
     }
