@@ -153,10 +153,10 @@ __device__ inline float tanh_d3(float h) {
   return -2.f * (d1 * d1 - 2.f * h * h * d1);
 }
 
-// Per-subdomain layer-1 tangent tables in LDS (3 * XD * H floats): with
+// Per-subdomain layer-1 tangent tables in LDS (4 * XD * H floats): with
 // zt_i[u] = W1[u][i] / sd_i,
 //   zt[i*H + u] = zt_i[u],  zt[XD*H + i*H + u] = -2 zt_i[u],
-//   zt[2*XD*H + i*H + u] = -2 zt_i[u]^2.
+//   zt[2*XD*H + i*H + u] = -2 zt_i[u]^2,  zt[3*XD*H + i*H + u] = zt_i[u]^2.
 template <int XD, int H, class SM>
 __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
   for (int t = threadIdx.x; t < XD * H; t += blockDim.x) {
@@ -169,6 +169,7 @@ __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
     zt[t] = z;
     zt[XD * H + t] = -2.f * z;
     zt[2 * XD * H + t] = -2.f * z * z;
+    zt[3 * XD * H + t] = z * z;
   }
 }
 
@@ -179,7 +180,7 @@ __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
 //   w1  = ss + W1 + 4g*XD      W1[16rb + 4g + r][i]        at [(16rb + r)*XD + i]
 //   vec = ss + 4g              B1/B2/WL[16rb + 4g + r]     at [off + 16rb + r]
 //   zt  = szt + 4g             zt[i][16rb + 4g + r]        at [i*H + 16rb + r]
-//   zm, zq: the -2 zt and -2 zt^2 tables, same indexing
+//   zm, zq, z2: the -2 zt, -2 zt^2 and zt^2 tables, same indexing
 //   wt  = swt + 4g(H+4) + pl   W2[16ob + pl][16kb + 4g + ks] at [(16kb + ks)(H+4) + 16ob]
 //   wr  = swr + 4g(H+4) + pl   W2[16ob + 4g + ks][16ib + pl] at [(16ob + ks)(H+4) + 16ib]
 struct LaneLds {
@@ -188,6 +189,7 @@ struct LaneLds {
   const float* zt;
   const float* zm;
   const float* zq;
+  const float* z2;
   const float* wt;
   const float* wr;
 };
@@ -201,6 +203,7 @@ __device__ inline LaneLds lane_lds(const float* ss, const float* szt, const floa
   v.zt = szt + 4 * g;
   v.zm = szt + XD * H + 4 * g;
   v.zq = szt + 2 * XD * H + 4 * g;
+  v.z2 = szt + 3 * XD * H + 4 * g;
   v.wt = swt + 4 * g * (H + 4) + pl;
   v.wr = swr + 4 * g * (H + 4) + pl;
   return v;
@@ -301,6 +304,42 @@ __device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H /
   }
 }
 
+// One layer-2 MFMA pass with the B operand given in registers (no bias):
+// Z[ob] = W2 . b, b[kb][ks] the value for unit 16kb + 4g + ks of this lane's pair.
+template <int H>
+__device__ inline void layer2_pass_raw(const LaneLds& v, const float (*b)[4], f4* Z) {
+  constexpr int NRB = H / 16;
+#pragma unroll
+  for (int ob = 0; ob < NRB; ++ob) Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < NRB; ++kb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(v.wt[(16 * kb + ks) * (H + 4) + 16 * ob], b[kb][ks], Z[ob]);
+}
+
+// The pair's layer-1 combination of every tangent the upstream gradient
+// asks for, s = sum_i (gur_i a1_i + gur_ii a1_ii) with a1_i = e1 zt_i and
+// a1_ii = e2 zt_i^2 (e1 = 1 - h1^2, e2 = -2 h1 e1):
+//   s = e1 (A - 2 h1 B),  A = sum_i gur_i zt_i,  B = sum_i gur_ii zt_i^2.
+template <class S, int H>
+__device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const float* gur, float (*sv)[4]) {
+#pragma unroll
+  for (int rb = 0; rb < H / 16; ++rb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float A = 0.f, B = 0.f;
+#pragma unroll
+      for (int i = 0; i < S::XD; ++i) {
+        if (S::has1(i)) A = fmaf(gur[S::col1(i)], v.zt[i * H + 16 * rb + r], A);
+        if (S::has2(i)) B = fmaf(gur[S::col2(i)], v.z2[i * H + 16 * rb + r], B);
+      }
+      const float h = h1[rb][r];
+      sv[rb][r] = fmaf(-h, h, 1.f) * fmaf(-2.f * h, B, A);
+    }
+}
+
 // ------------------------------------------------------------------ forward
 
 template <int XD, int H, int NH, int D1M, int D2M>
@@ -312,7 +351,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   using SM = Small<XD, H, NH>;
   constexpr int J = S::J, NRB = H / 16;
   __shared__ float ss[SM::N];
-  __shared__ float szt[3 * XD * H];
+  __shared__ float szt[4 * XD * H];
   __shared__ float swt[NH == 2 ? H * (H + 4) : 1];
   __shared__ float swr[NH == 2 && S::N2 > 0 ? H * (H + 4) : 1];
 
@@ -539,7 +578,7 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   __shared__ float sacc[L::P];
   __shared__ float ssmall[4 * NS * H];
   __shared__ float sbl[64];
-  __shared__ float szt[3 * XD * H];
+  __shared__ float szt[4 * XD * H];
 
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
@@ -724,7 +763,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   static_assert(H >= 16 && H <= 64, "hidden width 16, 32 or 64");
   static_assert(2 * O::TSZ >= L::P, "the gradient row reuses the transpose buffers");
   __shared__ float ss[SM::N];
-  __shared__ float szt[3 * XD * H];
+  __shared__ float szt[4 * XD * H];
   __shared__ float swt[H * TP];
   __shared__ float swr[H * TP];
   __shared__ __attribute__((aligned(16))) float strans[2 * O::TSZ];
@@ -801,8 +840,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       store_tile<H>(stbw, hv);
     }
 
-    // ---- column 0 of layer 2 (value): Z_0, h2, start of G_0
-    f4 G0[NRB];
+    // ---- Z_0 = W2 h1 + b2 -> h2
     float h2[NRB][4];
     {
       f4 Z0[NRB];
@@ -810,12 +848,42 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
+        for (int r = 0; r < 4; ++r) h2[ob][r] = tanh_f(Z0[ob][r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ---- S = W2 s;  G_0 = f1 gur_0 + f2 S (+ per axis),  dWL = gur_0 h2 + f1 S (+ per axis),
+    //      G_S = f1,  vt = W2~^T G_S (d L / d s)
+    f4 G0[NRB], vt[NRB];
+    if constexpr (S::N1 == 0) {
+      // value only: no tangents, s = 0
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const float h = tanh_f(Z0[ob][r]);
-          h2[ob][r] = h;
-          vWL[ob][r] = gur[0] * h;
+          const float h = h2[ob][r];
           G0[ob][r] = fmaf(-h, h, 1.f) * gur[0];
+          vWL[ob][r] = gur[0] * h;
         }
+    } else {
+      float sv[NRB][4];
+      pair_s<S, H>(v, h1, gur, sv);
+      f4 Sx[NRB];
+      layer2_pass_raw<H>(v, sv, Sx);
+      f4 gs[NRB];
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float h = h2[ob][r];
+          const float f1 = fmaf(-h, h, 1.f);
+          const float f2 = -2.f * (h * f1);
+          const float sx = Sx[ob][r];
+          G0[ob][r] = fmaf(f2, sx, f1 * gur[0]);
+          vWL[ob][r] = fmaf(f1, sx, gur[0] * h);
+          gs[ob][r] = f1;
+        }
+      w2t_pass<H>(v, gs, vt);
     }
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb)
@@ -823,100 +891,83 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       for (int r = 0; r < 4; ++r) gz10[rb][r] = 0.f;
     __builtin_amdgcn_sched_barrier(0);
 
-    // ---- per axis: Z_i, Z_ii -> G_i, G_ii -> dW2, W2^T G -> layer-1 terms
-    // Unrolled at h = 64; a real loop below that (one register allocation
-    // serves every axis), which the compiler allocates with fewer spills.
+    // ---- per axis.  d L / d a1_i = gur_i vt (+ W2~^T G_i when the axis has a
+    //      second derivative), d L / d a1_ii = gur_ii vt;  G_i = 2 gur_ii f2 Z_i
+    //      Unrolled at h = 64; a real loop below that (one register allocation
+    //      serves every axis), which the compiler allocates with fewer spills.
 #pragma unroll AXIS_UNROLL
     for (int i = 0; i < XD; ++i) {
       if (!S::has1(i)) continue;
       const bool two = S::has2(i);
       const float gur_a = pick<J>(gur, S::col1(i));
       const float gur_b = two ? pick<J>(gur, S::col2(i)) : 0.f;
-      const float gur_b2 = 2.f * gur_b;
       const float isd_i = pick<XD>(isd, i);
       const int zrow = i * H;
       opaque<NRB>(h1);
       opaque<NRB>(h2);
-      f4 Gi[NRB], Gii[NRB];
-      {
-        f4 Zi[NRB], Zii[NRB];
-        layer2_pass<H, SM>(v, h1, zrow, 1, Zi);
-        if (two) layer2_pass<H, SM>(v, h1, zrow, 2, Zii);
-#pragma unroll
-        for (int ob = 0; ob < NRB; ++ob)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            // G / WL (the WL factor is carried by the W2^T copy and by the
-            // final row scaling of dW2 and db2)
-            const float h = h2[ob][r];
-            const float d1 = fmaf(-h, h, 1.f);
-            const float d2 = -2.f * (h * d1);
-            const float zi = Zi[ob][r];
-            const float e = d2 * zi;
-            float dwl = fmaf(gur_a, d1 * zi, vWL[ob][r]);
-            float g0 = fmaf(e, gur_a, G0[ob][r]);
-            float gi = d1 * gur_a;
-            if (two) {
-              const float d3 = d1 * fmaf(-6.f, d1, 4.f);   // -2 (d1^2 + h d2)
-              const float zii = Zii[ob][r];
-              dwl = fmaf(gur_b, fmaf(e, zi, d1 * zii), dwl);
-              g0 = fmaf(fmaf(d3 * zi, zi, d2 * zii), gur_b, g0);
-              gi = fmaf(e, gur_b2, gi);
-              Gii[ob][r] = d1 * gur_b;
-            }
-            vWL[ob][r] = dwl;
-            G0[ob][r] = g0;
-            Gi[ob][r] = gi;
-          }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      const float zc = szt[zrow + ucol];
       float gzi[NRB][4];
-      // column i: publish G_i, then W2^T G_i while the other waves catch up
-      store_tile<H>(stgw, Gi);
-      {
-        f4 GA[NRB];
-        w2t_pass<H>(v, Gi, GA);
-#pragma unroll
-        for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float h = h1[rb][r];
-            const float d1 = fmaf(-h, h, 1.f);
-            const float t = (h * d1) * v.zm[zrow + 16 * rb + r];   // d2 zt
-            gzi[rb][r] = d1 * GA[rb][r];
-            gz10[rb][r] = fmaf(t, GA[rb][r], gz10[rb][r]);
-          }
-      }
-      __syncthreads();
-      dw2_shared<H, 1>(stgl, stbl, zc, dW2, wave);
-      __syncthreads();
       if (two) {
-        store_tile<H>(stgw, Gii);
+        f4 Gi[NRB];
+        {
+          f4 Zi[NRB];
+          layer2_pass<H, SM>(v, h1, zrow, 1, Zi);
+          const float gb2 = 2.f * gur_b;
+#pragma unroll
+          for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float h = h2[ob][r];
+              const float f1 = fmaf(-h, h, 1.f);
+              const float f2 = -2.f * (h * f1);
+              const float f3 = f1 * fmaf(-6.f, f1, 4.f);   // -2 (f1^2 + h f2)
+              const float z = Zi[ob][r];
+              const float bz = gur_b * z;
+              G0[ob][r] = fmaf(f3 * bz, z, G0[ob][r]);
+              vWL[ob][r] = fmaf(f2 * bz, z, vWL[ob][r]);
+              Gi[ob][r] = gb2 * (f2 * z);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        // column i: publish G_i, then W2~^T G_i while the other waves catch up
+        store_tile<H>(stgw, Gi);
         {
           f4 GA[NRB];
-          w2t_pass<H>(v, Gii, GA);
+          w2t_pass<H>(v, Gi, GA);
 #pragma unroll
           for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const float h = h1[rb][r];
-              const float d1 = fmaf(-h, h, 1.f);
-              const float t = (h * d1) * v.zm[zrow + 16 * rb + r];   // d2 zt
-              // d3 zt^2 = d1 (3 d1 - 2) (-2 zt^2)
-              const float s3 = (d1 * fmaf(3.f, d1, -2.f)) * v.zq[zrow + 16 * rb + r];
-              gzi[rb][r] = fmaf(t + t, GA[rb][r], gzi[rb][r]);
-              gz10[rb][r] = fmaf(s3, GA[rb][r], gz10[rb][r]);
+              const float e1 = fmaf(-h, h, 1.f);
+              const float he = h * e1;
+              const float tz = he * v.zm[zrow + 16 * rb + r];                        // e2 zt_i
+              const float t3 = (e1 * fmaf(-6.f, e1, 4.f)) * v.z2[zrow + 16 * rb + r];  // e3 zt_i^2
+              const float da = fmaf(gur_a, vt[rb][r], GA[rb][r]);
+              const float db = gur_b * vt[rb][r];
+              gzi[rb][r] = fmaf(tz + tz, db, e1 * da);
+              gz10[rb][r] = fmaf(t3, db, fmaf(tz, da, gz10[rb][r]));
             }
         }
         __syncthreads();
-        dw2_shared<H, 2>(stgl, stbl, szt[2 * XD * H + zrow + ucol], dW2, wave);
+        dw2_shared<H, 1>(stgl, stbl, szt[zrow + ucol], dW2, wave);
         __syncthreads();
+      } else {
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float h = h1[rb][r];
+            const float e1 = fmaf(-h, h, 1.f);
+            const float tz = (h * e1) * v.zm[zrow + 16 * rb + r];   // e2 zt_i
+            const float da = gur_a * vt[rb][r];
+            gzi[rb][r] = e1 * da;
+            gz10[rb][r] = fmaf(tz, da, gz10[rb][r]);
+          }
       }
       // dW1[:, i] through z1_i = zt_i (the 1 / sd_i factor after the sum)
       reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
     }
-    // ---- column 0: db2 += G_0, dW2 += G_0 h^T, GA_0 = W2^T G_0
+    // ---- column 0: db2 += G_0, dW2 += G_0 h^T, W2~^T G_0 -> layer 1
     opaque<NRB>(h1);
 #pragma unroll
     for (int ob = 0; ob < NRB; ++ob)
@@ -934,7 +985,24 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __syncthreads();
     dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
-    // (the next tile's leading barrier orders these reads before new writes)
+    __syncthreads();
+    // ---- column S: dW2 += G_S s^T (s and G_S = f1 rebuilt from h1 and h2)
+    if constexpr (S::N1 > 0) {
+      float sv[NRB][4];
+      pair_s<S, H>(v, h1, gur, sv);
+      f4 sw[NRB], gs[NRB];
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob) {
+        sw[ob] = f4{sv[ob][0], sv[ob][1], sv[ob][2], sv[ob][3]};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gs[ob][r] = fmaf(-h2[ob][r], h2[ob][r], 1.f);
+      }
+      store_tile<H>(stbw, sw);
+      store_tile<H>(stgw, gs);
+      __syncthreads();
+      dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
+      // (the next tile's leading barrier orders these reads before new writes)
+    }
 
     // dW1[:, i] += gz1_0 xh_i (the gz1_i parts were added per axis)
 #pragma unroll
