@@ -353,7 +353,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   __shared__ float ss[SM::N];
   __shared__ float szt[4 * XD * H];
   __shared__ float swt[NH == 2 ? H * (H + 4) : 1];
-  __shared__ float swr[NH == 2 && S::N2 > 0 ? H * (H + 4) : 1];
+  __shared__ float swr[NH == 2 && S::N1 > 0 ? H * (H + 4) : 1];
 
   const int chunk = blockIdx.x;
   const int model = wl.chunk_model[chunk];
@@ -361,7 +361,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
   stage_small<XD, H, NH>(th, ss);
-  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, S::N2 > 0 ? swr : nullptr);
+  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, S::N1 > 0 ? swr : nullptr);
   Dom d;
   load_dom(dom, model, d);
   float mu[XD], isd[XD];
@@ -444,10 +444,11 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 #pragma unroll
       for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
       __builtin_amdgcn_sched_barrier(0);
-      // sum_o WL d1 Z_ii = sum_u vt[u] a1_ii[u] with vt = W2^T (WL d1): one
-      // W2^T pass per sub-tile replaces the Z_ii passes of every axis
+      // sum_o WL d1 Z_c = sum_u vt[u] a1_c[u] with vt = W2^T (WL d1): one
+      // W2^T pass per sub-tile replaces the Z_ii passes of every axis and the
+      // Z_i passes of axes without a second derivative
       f4 vt[NRB];
-      if constexpr (S::N2 > 0) {
+      if constexpr (S::N1 > 0) {
 #pragma unroll
         for (int ib = 0; ib < NRB; ++ib) vt[ib] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -462,6 +463,17 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 #pragma unroll
       for (int i = 0; i < XD; ++i) {
         if (!S::has1(i)) continue;
+        if (!S::has2(i)) {
+          // u_i = sum_u vt[u] a1_i[u], a1_i = e1 zt_i
+#pragma unroll
+          for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const float h = h1[rb][r];
+              ur[S::col1(i)] = fmaf(vt[rb][r], fmaf(-h, h, 1.f) * v.zt[i * H + 16 * rb + r], ur[S::col1(i)]);
+            }
+          continue;
+        }
         f4 Zi[NRB];
         layer2_pass_rf<H, SM>(v, wf, h1, i * H, 1, Zi);
 #pragma unroll
@@ -470,9 +482,9 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
           for (int r = 0; r < 4; ++r) {
             const float zi = Zi[ob][r];
             ur[S::col1(i)] = fmaf(wd1[ob][r], zi, ur[S::col1(i)]);
-            if (S::has2(i)) ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
+            ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
           }
-        if (S::has2(i)) {
+        {
           // + sum_u vt[u] a1_ii[u], a1_ii = d2(h1) zt_i^2 = h d1 (-2 zt_i^2)
 #pragma unroll
           for (int rb = 0; rb < NRB; ++rb)
@@ -855,6 +867,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     // ---- S = W2 s;  G_0 = f1 gur_0 + f2 S (+ per axis),  dWL = gur_0 h2 + f1 S (+ per axis),
     //      G_S = f1,  vt = W2~^T G_S (d L / d s)
     f4 G0[NRB], vt[NRB];
+    float sv[NRB][4];   // s, kept for the column-S phase
     if constexpr (S::N1 == 0) {
       // value only: no tangents, s = 0
 #pragma unroll
@@ -866,7 +879,6 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           vWL[ob][r] = gur[0] * h;
         }
     } else {
-      float sv[NRB][4];
       pair_s<S, H>(v, h1, gur, sv);
       f4 Sx[NRB];
       layer2_pass_raw<H>(v, sv, Sx);
@@ -986,10 +998,8 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     __syncthreads();
     dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
     __syncthreads();
-    // ---- column S: dW2 += G_S s^T (s and G_S = f1 rebuilt from h1 and h2)
+    // ---- column S: dW2 += G_S s^T (G_S = f1 rebuilt from h2)
     if constexpr (S::N1 > 0) {
-      float sv[NRB][4];
-      pair_s<S, H>(v, h1, gur, sv);
       f4 sw[NRB], gs[NRB];
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob) {
