@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true", help="launch the step eagerly (for counter passes)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default chunk size (counter passes see one launch shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
     a = ap.parse_args()
@@ -208,7 +210,7 @@ def main():
     for h in logger.handlers:
         h.setStream(sys.stderr)   # stdout carries only the JSON line
 
-    c = configs.CONFIGS[a.config](graph=not a.no_graph)
+    c = configs.CONFIGS[a.config](graph=not a.no_graph, autotune=not a.no_autotune)
     log(f"config {a.config}: building problem")
     tr = FBPINNTrainer(c, write_outputs=False)
     eng, sched, x_test, _ = tr.setup(exact=False)

@@ -529,6 +529,12 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 //     gz1_0 = d1 GA_0 + sum_i d2 zt_i GA_i + sum_i d3 zt_i^2 GA_ii
 //     gz1_i = d1 GA_i + 2 d2 zt_i GA_ii
 //     dW1[u, k] += gz1_0 xh_k + gz1_k / sd_k,  db1 += gz1_0.
+// The two-layer kernel does not form G_i and G_ii column by column: every
+// linear use of the layer-1 tangents collapses into s = sum_i (gur_i a1_i +
+// gur_ii a1_ii) (pair_s), so with S = W2 s and vt = W2^T (WL f1)
+//   G_0 gains f2 S, the G_i a1_i^T and G_ii a1_ii^T columns of dW2 become
+//   (WL f1) s^T plus 2 gur_ii WL f2 Z_i a1_i^T per second-derivative axis,
+//   and GA_i = gur_i vt (+ W2^T of that axis's column), GA_ii = gur_ii vt.
 // The two-layer kernel carries G~ = G / WL (row-wise; no division is
 // performed: G~ is formed without the WL factor) and restores WL where G is
 // consumed: W2^T G = (diag(WL) W2)^T G~ through a row-scaled LDS copy of W2,
