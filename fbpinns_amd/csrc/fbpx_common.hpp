@@ -71,22 +71,48 @@ __device__ inline void load_dom(const float* dom, int model, Dom& d) {
   d.flag = r[10];
 }
 
-// tanh(z) = sign(z) (1 - e) / (1 + e), e = exp(-2|z|), with the hardware
-// exponential and reciprocal; branch-free.  Absolute error <= 1.2e-7 (the
-// 1 - e cancellation near 0 costs relative, not absolute, accuracy).
+// tanh(z) = 1 - 2 / (1 + e^{2z}) with the hardware base-2 exponential and
+// reciprocal: five instructions, branch-free, exact limits (e^{2z} -> inf
+// gives 1, -> 0 gives -1).  Absolute error <= 2.5e-7; the rounding of
+// 1 - 2r near z = 0 costs relative, not absolute, accuracy.
 __device__ inline float tanh_f(float z) {
-  const float e = __expf(-2.f * fabsf(z));
-  const float t = (1.f - e) * __builtin_amdgcn_rcpf(1.f + e);
-  return copysignf(t, z);
+  constexpr float k2log2e = 2.8853900817779268f;   // 2 / ln 2
+  const float e = __builtin_amdgcn_exp2f(z * k2log2e);
+  return fmaf(-2.f, __builtin_amdgcn_rcpf(1.f + e), 1.f);
 }
 
 __device__ inline f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// cos and sin of a half angle |phi| <= pi/2 (no range reduction): even and
+// odd polynomials of degree 10 and 11, least-squares fits in phi^2 on
+// Chebyshev nodes refined toward equal ripple.  With these float32
+// coefficients the error is < 2.1e-8 in exact arithmetic and < 1.5e-7
+// absolute in float32 evaluation.
+__device__ inline void half_angle_cs(float phi, float& c, float& s) {
+  const float y = phi * phi;
+  float pc = -2.6051492e-07f;
+  pc = fmaf(pc, y, 2.4760162e-05f);
+  pc = fmaf(pc, y, -1.3888362e-03f);
+  pc = fmaf(pc, y, 4.1666638e-02f);
+  pc = fmaf(pc, y, -0.5f);
+  c = fmaf(pc, y, 1.f);
+  float ps = -2.3888729e-08f;
+  ps = fmaf(ps, y, 2.7525236e-06f);
+  ps = fmaf(ps, y, -1.9840860e-04f);
+  ps = fmaf(ps, y, 8.3333310e-03f);
+  ps = fmaf(ps, y, -1.6666667e-01f);
+  s = phi * fmaf(ps, y, 1.f);
+}
+
 // Window jets of one pair (windows.py:25-35 with decompositions.py:196-199).
 // Pairs only exist for points inside the box (inclusive), where both
-// heaviside factors equal 1 and have zero derivative.
+// heaviside factors equal 1 and have zero derivative, and the cosine angle
+// theta = pi (x - mu) / sd lies in [-pi, pi].  With phi = theta / 2,
+// c = cos(phi), s = sin(phi) (half-angle identities, k = pi / sd):
+//   ((1 + cos theta) / 2)^2 = c^4,   d/dx = -2 k s c^3,
+//   d2/dx2 = k^2 (sin^2 theta - cos theta (1 + cos theta)) / 2 = k^2 c^2 (3 - 4 c^2).
 template <class S>
 __device__ inline void window_jets(const float* x, const Dom& d, float* w) {
   constexpr int XD = S::XD;
@@ -96,13 +122,14 @@ __device__ inline void window_jets(const float* x, const Dom& d, float* w) {
     const float mu = (d.xmin[i] + d.xmax[i]) * 0.5f;
     const float sd = (d.xmax[i] - d.xmin[i]) * 0.5f;
     const float k = kPi / sd;
-    const float th = (x[i] - mu) * k;
-    float s, co;
-    sincosf(th, &s, &co);
-    const float hc = (1.f + co) * 0.5f;
-    c[i] = hc * hc;
-    c1[i] = -k * (1.f + co) * s * 0.5f;
-    c2[i] = k * k * (s * s - co * (1.f + co)) * 0.5f;
+    // clamped so that rounding at the box edge cannot leave the fitted range
+    const float phi = __builtin_amdgcn_fmed3f((x[i] - mu) * (0.5f * k), -0.5f * kPi, 0.5f * kPi);
+    float co, si;
+    half_angle_cs(phi, co, si);
+    const float cc = co * co;
+    c[i] = cc * cc;
+    c1[i] = (-2.f * k) * (si * (cc * co));
+    c2[i] = (k * k) * (cc * fmaf(-4.f, cc, 3.f));
   }
   float prod = 1.f;
 #pragma unroll

@@ -273,7 +273,8 @@ __device__ inline void load_w2_frag(const LaneLds& v, float (*wf)[4][H / 16]) {
       for (int ob = 0; ob < H / 16; ++ob) wf[kb][ks][ob] = v.wt[(16 * kb + ks) * (H + 4) + 16 * ob];
 }
 
-// layer2_pass with the weights taken from register fragments.
+// layer2_pass with the weights taken from register fragments; kind 3:
+// B = base zt_i with base = d1 of layer 1 supplied in place of h.
 template <int H, class SM>
 __device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H / 16], const float (*h1)[4], int zrow,
                                       int kind, f4* Z) {
@@ -297,6 +298,8 @@ __device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H /
         b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];
       } else if (kind == 2) {
         b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];
+      } else if (kind == 3) {
+        b = h * v.zt[zrow + 16 * kb + ks];                           // (given d1) zt
       }
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(wf[kb][ks][ob], b, Z[ob]);
@@ -321,10 +324,13 @@ __device__ inline void layer2_pass_raw(const LaneLds& v, const float (*b)[4], f4
 
 // The pair's layer-1 combination of every tangent the upstream gradient
 // asks for, s = sum_i (gur_i a1_i + gur_ii a1_ii) with a1_i = e1 zt_i and
-// a1_ii = e2 zt_i^2 (e1 = 1 - h1^2, e2 = -2 h1 e1):
-//   s = e1 (A - 2 h1 B),  A = sum_i gur_i zt_i,  B = sum_i gur_ii zt_i^2.
+// a1_ii = e2 zt_i^2 (e1 = 1 - h1^2, e2 = -2 h1 e1, e3 = e1 (4 - 6 e1)):
+//   s = e1 (A - 2 h1 B),  A = sum_i gur_i zt_i,  B = sum_i gur_ii zt_i^2,
+// and the factor that the W2^T vector vt multiplies in the layer-1 gradient,
+//   rv = e2 A + e3 B = e1 (-2 h1 A + (4 - 6 e1) B).
 template <class S, int H>
-__device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const float* gur, float (*sv)[4]) {
+__device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const float* gur, float (*sv)[4],
+                              float (*rv)[4]) {
 #pragma unroll
   for (int rb = 0; rb < H / 16; ++rb)
 #pragma unroll
@@ -336,7 +342,9 @@ __device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const floa
         if (S::has2(i)) B = fmaf(gur[S::col2(i)], v.z2[i * H + 16 * rb + r], B);
       }
       const float h = h1[rb][r];
-      sv[rb][r] = fmaf(-h, h, 1.f) * fmaf(-2.f * h, B, A);
+      const float e1 = fmaf(-h, h, 1.f);
+      sv[rb][r] = e1 * fmaf(-2.f * h, B, A);
+      rv[rb][r] = e1 * fmaf(h * A, -2.f, fmaf(-6.f, e1, 4.f) * B);
     }
 }
 
@@ -427,8 +435,10 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
       f4 Z0[NRB];
       layer2_pass_rf<H, SM>(v, wf, h1, 0, 0, Z0);
       // output-layer weights times the layer-2 tanh derivatives, per element:
-      // wd1 = WL d1, wd2 = WL d2, so that u_i = sum WL d1 Z_i and the Z_i^2
-      // part of u_ii = sum WL (d1 Z_ii + d2 Z_i^2) cost one or two FMAs
+      // wd1 = WL f1 and wd2 = WL f1 h2 = WL f2 / (-2), so that u_i = sum wd1 Z_i
+      // and the Z_i^2 part of u_ii = sum WL (f1 Z_ii + f2 Z_i^2) costs two FMAs;
+      // every second-derivative column is accumulated without its factor -2,
+      // which is applied once to the pair's totals
       float wd1[NRB][4], wd2[NRB][4];
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob)
@@ -436,19 +446,21 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
         for (int r = 0; r < 4; ++r) {
           const float wlu = v.vec[SM::WL + 16 * ob + r];
           const float h = tanh_f(Z0[ob][r]);
-          const float d1 = fmaf(-h, h, 1.f);
           ur[0] = fmaf(wlu, h, ur[0]);
-          wd1[ob][r] = wlu * d1;
-          wd2[ob][r] = (-2.f * wlu) * (h * d1);
+          wd1[ob][r] = wlu * fmaf(-h, h, 1.f);
+          wd2[ob][r] = wd1[ob][r] * h;
         }
 #pragma unroll
       for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
       __builtin_amdgcn_sched_barrier(0);
-      // sum_o WL d1 Z_c = sum_u vt[u] a1_c[u] with vt = W2^T (WL d1): one
+      // sum_o WL f1 Z_c = sum_u vt[u] a1_c[u] with vt = W2^T (WL f1): one
       // W2^T pass per sub-tile replaces the Z_ii passes of every axis and the
-      // Z_i passes of axes without a second derivative
-      f4 vt[NRB];
+      // Z_i passes of axes without a second derivative.  With e1 = 1 - h1^2,
+      // a1_i = e1 zt_i and a1_ii = -2 h1 e1 zt_i^2, so the per-axis dot
+      // products take ve = vt e1 (first order) and w = vt e1 h1 (second order).
+      float e1[NRB][4], ve[NRB][4], w[NRB][4];
       if constexpr (S::N1 > 0) {
+        f4 vt[NRB];
 #pragma unroll
         for (int ib = 0; ib < NRB; ++ib) vt[ib] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -459,23 +471,29 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
             for (int ib = 0; ib < NRB; ++ib)
               vt[ib] = mfma16(v.wr[(16 * ob + ks) * (H + 4) + 16 * ib], wd1[ob][ks], vt[ib]);
         __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float h = h1[rb][r];
+            e1[rb][r] = fmaf(-h, h, 1.f);
+            ve[rb][r] = vt[rb][r] * e1[rb][r];
+            w[rb][r] = ve[rb][r] * h;
+          }
       }
 #pragma unroll
       for (int i = 0; i < XD; ++i) {
         if (!S::has1(i)) continue;
         if (!S::has2(i)) {
-          // u_i = sum_u vt[u] a1_i[u], a1_i = e1 zt_i
+          // u_i = sum_u vt[u] e1[u] zt_i[u]
 #pragma unroll
           for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const float h = h1[rb][r];
-              ur[S::col1(i)] = fmaf(vt[rb][r], fmaf(-h, h, 1.f) * v.zt[i * H + 16 * rb + r], ur[S::col1(i)]);
-            }
+            for (int r = 0; r < 4; ++r) ur[S::col1(i)] = fmaf(ve[rb][r], v.zt[i * H + 16 * rb + r], ur[S::col1(i)]);
           continue;
         }
         f4 Zi[NRB];
-        layer2_pass_rf<H, SM>(v, wf, h1, i * H, 1, Zi);
+        layer2_pass_rf<H, SM>(v, wf, e1, i * H, 3, Zi);
 #pragma unroll
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
@@ -484,17 +502,11 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
             ur[S::col1(i)] = fmaf(wd1[ob][r], zi, ur[S::col1(i)]);
             ur[S::col2(i)] = fmaf(wd2[ob][r] * zi, zi, ur[S::col2(i)]);
           }
-        {
-          // + sum_u vt[u] a1_ii[u], a1_ii = d2(h1) zt_i^2 = h d1 (-2 zt_i^2)
+        // + sum_u vt[u] h1 e1 zt_i^2 (the a1_ii column, factor -2 deferred)
 #pragma unroll
-          for (int rb = 0; rb < NRB; ++rb)
+        for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const float h = h1[rb][r];
-              const float a = (h * fmaf(-h, h, 1.f)) * v.zq[i * H + 16 * rb + r];
-              ur[S::col2(i)] = fmaf(vt[rb][r], a, ur[S::col2(i)]);
-            }
-        }
+          for (int r = 0; r < 4; ++r) ur[S::col2(i)] = fmaf(w[rb][r], v.z2[i * H + 16 * rb + r], ur[S::col2(i)]);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -503,7 +515,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
     for (int c = 0; c < J; ++c) uj[c] = sum_groups(ur[c]);
     uj[0] = (uj[0] + ss[SM::BL]) * d.usd + d.umu;
 #pragma unroll
-    for (int c = 1; c < J; ++c) uj[c] *= d.usd;
+    for (int c = 1; c < J; ++c) uj[c] *= (NH == 2 && c > S::N1) ? -2.f * d.usd : d.usd;
     float wj[J], uw[J];
     window_jets<S>(xv, d, wj);
     jet_mul<S>(uj, wj, uw);
@@ -870,10 +882,15 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __builtin_amdgcn_sched_barrier(0);
 
-    // ---- S = W2 s;  G_0 = f1 gur_0 + f2 S (+ per axis),  dWL = gur_0 h2 + f1 S (+ per axis),
-    //      G_S = f1,  vt = W2~^T G_S (d L / d s)
+    // ---- S = W2 s,  G_S = f1,  vt = W2~^T G_S (d L / d s).  With
+    //      Q = sum_i gur_ii Z_i^2 (accumulated per axis):
+    //        G_0 = f1 gur_0 + f2 S + f3 Q,  dWL = gur_0 h2 + f1 S + f2 Q
+    //      (f1 = 1 - h2^2, f2 = -2 h2 f1, f3 = f1 (4 - 6 f1)).  The layer-1
+    //      gradient starts as gz1_0 = vt rv (see pair_s); the axes add
+    //      e2 zt_i GA_i and column 0 adds e1 GA_0.
     f4 G0[NRB], vt[NRB];
     float sv[NRB][4];   // s, kept for the column-S phase
+    float Sx[NRB][4], Q[NRB][4];
     if constexpr (S::N1 == 0) {
       // value only: no tangents, s = 0
 #pragma unroll
@@ -883,34 +900,39 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           const float h = h2[ob][r];
           G0[ob][r] = fmaf(-h, h, 1.f) * gur[0];
           vWL[ob][r] = gur[0] * h;
+          gz10[ob][r] = 0.f;
         }
     } else {
-      pair_s<S, H>(v, h1, gur, sv);
-      f4 Sx[NRB];
-      layer2_pass_raw<H>(v, sv, Sx);
+      float rv[NRB][4];
+      pair_s<S, H>(v, h1, gur, sv, rv);
+      {
+        f4 Sv[NRB];
+        layer2_pass_raw<H>(v, sv, Sv);
+#pragma unroll
+        for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            Sx[ob][r] = Sv[ob][r];
+            Q[ob][r] = 0.f;
+          }
+      }
       f4 gs[NRB];
 #pragma unroll
       for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float h = h2[ob][r];
-          const float f1 = fmaf(-h, h, 1.f);
-          const float f2 = -2.f * (h * f1);
-          const float sx = Sx[ob][r];
-          G0[ob][r] = fmaf(f2, sx, f1 * gur[0]);
-          vWL[ob][r] = fmaf(f1, sx, gur[0] * h);
-          gs[ob][r] = f1;
-        }
+        for (int r = 0; r < 4; ++r) gs[ob][r] = fmaf(-h2[ob][r], h2[ob][r], 1.f);
       w2t_pass<H>(v, gs, vt);
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gz10[rb][r] = vt[rb][r] * rv[rb][r];
     }
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) gz10[rb][r] = 0.f;
     __builtin_amdgcn_sched_barrier(0);
 
     // ---- per axis.  d L / d a1_i = gur_i vt (+ W2~^T G_i when the axis has a
     //      second derivative), d L / d a1_ii = gur_ii vt;  G_i = 2 gur_ii f2 Z_i
+    //        gz1_i = e1 (gur_i vt + GA_i) + 2 gur_ii e2 zt_i vt,
+    //        gz1_0 += e2 zt_i GA_i.
     //      Unrolled at h = 64; a real loop below that (one register allocation
     //      serves every axis), which the compiler allocates with fewer spills.
 #pragma unroll AXIS_UNROLL
@@ -929,20 +951,16 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         {
           f4 Zi[NRB];
           layer2_pass<H, SM>(v, h1, zrow, 1, Zi);
-          const float gb2 = 2.f * gur_b;
+          const float gm4 = -4.f * gur_b;
 #pragma unroll
           for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const float h = h2[ob][r];
-              const float f1 = fmaf(-h, h, 1.f);
-              const float f2 = -2.f * (h * f1);
-              const float f3 = f1 * fmaf(-6.f, f1, 4.f);   // -2 (f1^2 + h f2)
               const float z = Zi[ob][r];
-              const float bz = gur_b * z;
-              G0[ob][r] = fmaf(f3 * bz, z, G0[ob][r]);
-              vWL[ob][r] = fmaf(f2 * bz, z, vWL[ob][r]);
-              Gi[ob][r] = gb2 * (f2 * z);
+              Q[ob][r] = fmaf(gur_b * z, z, Q[ob][r]);
+              // 2 gur_b f2 z = -4 gur_b (h - h^3) z
+              Gi[ob][r] = gm4 * (fmaf(-h * h, h, h) * z);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -951,19 +969,17 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         {
           f4 GA[NRB];
           w2t_pass<H>(v, Gi, GA);
+          const float gb2 = 2.f * gur_b;
 #pragma unroll
           for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const float h = h1[rb][r];
               const float e1 = fmaf(-h, h, 1.f);
-              const float he = h * e1;
-              const float tz = he * v.zm[zrow + 16 * rb + r];                        // e2 zt_i
-              const float t3 = (e1 * fmaf(-6.f, e1, 4.f)) * v.z2[zrow + 16 * rb + r];  // e3 zt_i^2
-              const float da = fmaf(gur_a, vt[rb][r], GA[rb][r]);
-              const float db = gur_b * vt[rb][r];
-              gzi[rb][r] = fmaf(tz + tz, db, e1 * da);
-              gz10[rb][r] = fmaf(t3, db, fmaf(tz, da, gz10[rb][r]));
+              const float p = (h * e1) * v.zm[zrow + 16 * rb + r];   // e2 zt_i
+              const float q = fmaf(gb2, p, gur_a * e1);
+              gzi[rb][r] = fmaf(vt[rb][r], q, e1 * GA[rb][r]);
+              gz10[rb][r] = fmaf(p, GA[rb][r], gz10[rb][r]);
             }
         }
         __syncthreads();
@@ -973,17 +989,24 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float h = h1[rb][r];
-            const float e1 = fmaf(-h, h, 1.f);
-            const float tz = (h * e1) * v.zm[zrow + 16 * rb + r];   // e2 zt_i
-            const float da = gur_a * vt[rb][r];
-            gzi[rb][r] = e1 * da;
-            gz10[rb][r] = fmaf(tz, da, gz10[rb][r]);
-          }
+          for (int r = 0; r < 4; ++r) gzi[rb][r] = fmaf(-h1[rb][r], h1[rb][r], 1.f) * (gur_a * vt[rb][r]);
       }
       // dW1[:, i] through z1_i = zt_i (the 1 / sd_i factor after the sum)
       reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
+    }
+    if constexpr (S::N1 > 0) {
+      opaque<NRB>(h2);
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float h = h2[ob][r];
+          const float f1 = fmaf(-h, h, 1.f);
+          const float f2 = -2.f * (h * f1);
+          const float f3 = f1 * fmaf(-6.f, f1, 4.f);
+          G0[ob][r] = fmaf(f3, Q[ob][r], fmaf(f2, Sx[ob][r], f1 * gur[0]));
+          vWL[ob][r] = fmaf(f2, Q[ob][r], fmaf(f1, Sx[ob][r], gur[0] * h));
+        }
     }
     // ---- column 0: db2 += G_0, dW2 += G_0 h^T, W2~^T G_0 -> layer 1
     opaque<NRB>(h1);
