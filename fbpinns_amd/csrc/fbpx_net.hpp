@@ -51,20 +51,61 @@ __device__ inline void stage_small(const float* th, float* ss) {
     ss[i] = th[i < SM::B2 ? i : (i < SM::WL ? L::B2 + (i - SM::B2) : L::WL + (i - SM::WL))];
 }
 
-// Padded LDS copies of W2: wt[k*(H+4) + o] = W2[o][k] (read as the forward A
-// operand) and wr[o*(H+4) + k] = WL[o] W2[o][k] (read as the A operand of
-// (diag(WL) W2)^T; the backward carries G / WL, see bwd_two), or W2[o][k]
-// itself when wl is null.  The stride H+4
-// puts lane groups g = 0, 1 of a half-wave 16 banks apart, so both read
-// patterns are bank-conflict-free.
+// Lane-major LDS fragments of W2: the MFMA A operands of one K-step for all
+// 64 lanes, NRB = H/16 contiguous floats per lane (one ds_read_b128 at
+// H = 64).  With lane l = 16g + pl and K-step q:
+//   wt[(q*64 + l)*NRB + ob] = W2[16ob + pl][16kb + 4g + ks],        q = 4kb + ks
+//     (A of W2 . b: rows o = 16ob + pl, contraction units 16kb + 4g + ks)
+//   wr[(q*64 + l)*NRB + ib] = c_o W2[o][16ib + pl], o = 16ob + 4g + ks, q = 4ob + ks
+//     (A of (diag(c) W2)^T . g), c = WL (the backward carries G / WL, see
+//     bwd_two) or 1 when wl is null.
+// Consecutive lanes read consecutive 16-byte slots: conflict-free.
 template <int H>
 __device__ inline void stage_w2(const float* w2, const float* wl, float* wt, float* wr) {
-  for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
-    const int o = i / H, k = i % H;
-    const float v = w2[i];
-    if (wt) wt[k * (H + 4) + o] = v;
-    if (wr) wr[o * (H + 4) + k] = wl ? v * wl[o] : v;
+  constexpr int NRB = H / 16;
+  for (int t = threadIdx.x; t < 4 * NRB * 64; t += blockDim.x) {
+    const int q = t >> 6, l = t & 63, g = l >> 4, pl = l & 15;
+    const int kb = q >> 2, ks = q & 3;
+    const int k = 16 * kb + 4 * g + ks;   // contraction unit (wt) / row o (wr)
+    float a[NRB], c[NRB];
+    const float cw = (wr && wl) ? wl[k] : 1.f;
+#pragma unroll
+    for (int j = 0; j < NRB; ++j) {
+      if (wt) a[j] = w2[(16 * j + pl) * H + k];
+      if (wr) c[j] = w2[k * H + 16 * j + pl] * cw;
+    }
+#pragma unroll
+    for (int j = 0; j < NRB; ++j) {
+      if (wt) wt[t * NRB + j] = a[j];
+      if (wr) wr[t * NRB + j] = c[j];
+    }
   }
+}
+
+// One K-step of lane-major fragments (see stage_w2).
+template <int NRB>
+struct Frag {
+  float v[NRB];
+};
+
+template <int NRB>
+__device__ inline Frag<NRB> ld_frag(const float* p) {
+  Frag<NRB> f;
+  if constexpr (NRB == 4) {
+    const f4 t = *reinterpret_cast<const f4*>(p);
+    f.v[0] = t[0];
+    f.v[1] = t[1];
+    f.v[2] = t[2];
+    f.v[3] = t[3];
+  } else if constexpr (NRB == 2) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 t = *reinterpret_cast<const f2*>(p);
+    f.v[0] = t[0];
+    f.v[1] = t[1];
+  } else {
+    f.v[0] = p[0];
+  }
+  return f;
 }
 
 __device__ inline float sum_groups(float v) {
@@ -92,6 +133,49 @@ __device__ inline void split_stage(float* v, bool upper) {
   }
 }
 
+// The two mirror stages with whole DPP banks (4 lanes) on each side: lanes
+// of the banks in the low mask keep lo + partner's lo, the others hi +
+// partner's hi, written by two bank-masked DPP adds into one register (no
+// selects).  row_mirror pairs pl with 15 - pl (banks 0,1 | 2,3),
+// row_half_mirror pl with pl ^ 7 (banks 0,2 | 1,3).  One asm block per
+// stage; its leading s_nop covers the VALU-write -> DPP-read hazard on the
+// operands (the later pairs' operands are older still).
+#define FBPX_DPP_PAIR(CTRL, LO, HI, D, A, B)                               \
+  "v_add_f32_dpp %" #D ", %" #A ", %" #A " " CTRL " row_mask:0xf bank_mask:" LO "\n\t" \
+  "v_add_f32_dpp %" #D ", %" #B ", %" #B " " CTRL " row_mask:0xf bank_mask:" HI "\n\t"
+
+__device__ inline void mirror_stage(float* v) {
+  float d[8];
+  asm("s_nop 1\n\t"
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 0, 8, 16)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 1, 9, 17)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 2, 10, 18)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 3, 11, 19)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 4, 12, 20)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 5, 13, 21)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 6, 14, 22)
+      FBPX_DPP_PAIR("row_mirror", "0x3", "0xc", 7, 15, 23)
+      : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3]), "=&v"(d[4]), "=&v"(d[5]), "=&v"(d[6]), "=&v"(d[7])
+      : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]), "v"(v[5]), "v"(v[6]), "v"(v[7]),
+        "v"(v[8]), "v"(v[9]), "v"(v[10]), "v"(v[11]), "v"(v[12]), "v"(v[13]), "v"(v[14]), "v"(v[15]));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = d[k];
+}
+
+__device__ inline void half_mirror_stage(float* v) {
+  float d[4];
+  asm("s_nop 1\n\t"
+      FBPX_DPP_PAIR("row_half_mirror", "0x5", "0xa", 0, 4, 8)
+      FBPX_DPP_PAIR("row_half_mirror", "0x5", "0xa", 1, 5, 9)
+      FBPX_DPP_PAIR("row_half_mirror", "0x5", "0xa", 2, 6, 10)
+      FBPX_DPP_PAIR("row_half_mirror", "0x5", "0xa", 3, 7, 11)
+      : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3])
+      : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]), "v"(v[5]), "v"(v[6]), "v"(v[7]));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = d[k];
+}
+#undef FBPX_DPP_PAIR
+
 // vals[rb][r] holds this lane's contribution for unit 16rb + 4g + r.  Sums
 // every entry over the 16 pair-lanes of the row and adds the total for unit
 // 16*(pl/4) + 4g + pl%4 into dst (lane pl owns that unit; no two lanes of a
@@ -117,8 +201,8 @@ __device__ inline void reduce_units_to_lds(const float (*vals)[4], float* dst, i
 #pragma unroll
     for (int k = 0; k < V; ++k) v[k] += dpp<0x141>(v[k]);
   }
-  if constexpr (V == 16) split_stage<16, 0x140>(v, (pl >> 3) & 1);
-  if constexpr (V >= 8) split_stage<8, 0x141>(v, (pl >> 2) & 1);
+  if constexpr (V == 16) mirror_stage(v);
+  if constexpr (V >= 8) half_mirror_stage(v);
   split_stage<4, 0x1B>(v, (pl >> 1) & 1);
   split_stage<2, 0xB1>(v, pl & 1);
   if (pl < V) dst[16 * (pl >> 2) + 4 * g + (pl & 3)] += v[0] * post;
@@ -181,8 +265,8 @@ __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
 //   vec = ss + 4g              B1/B2/WL[16rb + 4g + r]     at [off + 16rb + r]
 //   zt  = szt + 4g             zt[i][16rb + 4g + r]        at [i*H + 16rb + r]
 //   zm, zq, z2: the -2 zt, -2 zt^2 and zt^2 tables, same indexing
-//   wt  = swt + 4g(H+4) + pl   W2[16ob + pl][16kb + 4g + ks] at [(16kb + ks)(H+4) + 16ob]
-//   wr  = swr + 4g(H+4) + pl   W2[16ob + 4g + ks][16ib + pl] at [(16ob + ks)(H+4) + 16ib]
+//   wt  = swt + lane*NRB       K-step q of the W2 fragments at [q*64*NRB]
+//   wr  = swr + lane*NRB       K-step q of the transposed fragments, same
 struct LaneLds {
   const float* w1;
   const float* vec;
@@ -204,8 +288,8 @@ __device__ inline LaneLds lane_lds(const float* ss, const float* szt, const floa
   v.zm = szt + XD * H + 4 * g;
   v.zq = szt + 2 * XD * H + 4 * g;
   v.z2 = szt + 3 * XD * H + 4 * g;
-  v.wt = swt + 4 * g * (H + 4) + pl;
-  v.wr = swr + 4 * g * (H + 4) + pl;
+  v.wt = swt + (16 * g + pl) * (H / 16);
+  v.wr = swr + (16 * g + pl) * (H / 16);
   return v;
 }
 
@@ -244,6 +328,7 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
   for (int kb = 0; kb < NRB; ++kb) {
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
+      const Frag<NRB> w = ld_frag<NRB>(v.wt + (4 * kb + ks) * 64 * NRB);
       const float h = h1[kb][ks];
       float b = h;
       if (kind == 1) {
@@ -252,10 +337,7 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
         b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];      // d2 zt^2 = h d1 (-2 zt^2)
       }
 #pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) {
-        const float w = v.wt[(16 * kb + ks) * (H + 4) + 16 * ob];
-        Z[ob] = mfma16(w, b, Z[ob]);
-      }
+      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(w.v[ob], b, Z[ob]);
     }
   }
 }
@@ -268,9 +350,11 @@ __device__ inline void load_w2_frag(const LaneLds& v, float (*wf)[4][H / 16]) {
 #pragma unroll
   for (int kb = 0; kb < H / 16; ++kb)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+    for (int ks = 0; ks < 4; ++ks) {
+      const Frag<H / 16> w = ld_frag<H / 16>(v.wt + (4 * kb + ks) * 64 * (H / 16));
 #pragma unroll
-      for (int ob = 0; ob < H / 16; ++ob) wf[kb][ks][ob] = v.wt[(16 * kb + ks) * (H + 4) + 16 * ob];
+      for (int ob = 0; ob < H / 16; ++ob) wf[kb][ks][ob] = w.v[ob];
+    }
 }
 
 // layer2_pass with the weights taken from register fragments; kind 3:
@@ -317,9 +401,11 @@ __device__ inline void layer2_pass_raw(const LaneLds& v, const float (*b)[4], f4
 #pragma unroll
   for (int kb = 0; kb < NRB; ++kb)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+    for (int ks = 0; ks < 4; ++ks) {
+      const Frag<NRB> w = ld_frag<NRB>(v.wt + (4 * kb + ks) * 64 * NRB);
 #pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(v.wt[(16 * kb + ks) * (H + 4) + 16 * ob], b[kb][ks], Z[ob]);
+      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(w.v[ob], b[kb][ks], Z[ob]);
+    }
 }
 
 // The pair's layer-1 combination of every tangent the upstream gradient
@@ -351,7 +437,7 @@ __device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const floa
 // ------------------------------------------------------------------ forward
 
 template <int XD, int H, int NH, int D1M, int D2M>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, float* __restrict__ pairbuf) {
   using S = Schema<XD, D1M, D2M>;
@@ -360,8 +446,8 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   constexpr int J = S::J, NRB = H / 16;
   __shared__ float ss[SM::N];
   __shared__ float szt[4 * XD * H];
-  __shared__ float swt[NH == 2 ? H * (H + 4) : 1];
-  __shared__ float swr[NH == 2 && S::N1 > 0 ? H * (H + 4) : 1];
+  __shared__ __attribute__((aligned(16))) float swt[NH == 2 ? H * H : 1];
+  __shared__ __attribute__((aligned(16))) float swr[NH == 2 && S::N1 > 0 ? H * H : 1];
 
   const int chunk = blockIdx.x;
   const int model = wl.chunk_model[chunk];
@@ -466,10 +552,11 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
 #pragma unroll
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
-          for (int ks = 0; ks < 4; ++ks)
+          for (int ks = 0; ks < 4; ++ks) {
+            const Frag<NRB> w = ld_frag<NRB>(v.wr + (4 * ob + ks) * 64 * NRB);
 #pragma unroll
-            for (int ib = 0; ib < NRB; ++ib)
-              vt[ib] = mfma16(v.wr[(16 * ob + ks) * (H + 4) + 16 * ib], wd1[ob][ks], vt[ib]);
+            for (int ib = 0; ib < NRB; ++ib) vt[ib] = mfma16(w.v[ib], wd1[ob][ks], vt[ib]);
+          }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
@@ -717,12 +804,11 @@ __device__ inline void w2t_pass(const LaneLds& v, const f4* G, f4* GA) {
 #pragma unroll
   for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
+    for (int ks = 0; ks < 4; ++ks) {
+      const Frag<NRB> w = ld_frag<NRB>(v.wr + (4 * ob + ks) * 64 * NRB);
 #pragma unroll
-      for (int ib = 0; ib < NRB; ++ib) {
-        const float w = v.wr[(16 * ob + ks) * (H + 4) + 16 * ib];
-        GA[ib] = mfma16(w, G[ob][ks], GA[ib]);
-      }
+      for (int ib = 0; ib < NRB; ++ib) GA[ib] = mfma16(w.v[ib], G[ob][ks], GA[ib]);
+    }
 }
 
 // Workgroup-shared dW2 accumulation.  The 4 waves of a workgroup process
@@ -787,15 +873,15 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   using L = Layout<XD, H, 2>;
   using SM = Small<XD, H, 2>;
   using O = DW2Own<H>;
-  constexpr int J = S::J, NRB = H / 16, TP = H + 4, NS = 3 + XD;
+  constexpr int J = S::J, NRB = H / 16, NS = 3 + XD;
   constexpr int K_WL = 0, K_B1 = 1, K_W1 = 2, K_B2 = 2 + XD;
   constexpr int AXIS_UNROLL = H == 64 ? XD : 1;
   static_assert(H >= 16 && H <= 64, "hidden width 16, 32 or 64");
   static_assert(2 * O::TSZ >= L::P, "the gradient row reuses the transpose buffers");
   __shared__ float ss[SM::N];
   __shared__ float szt[4 * XD * H];
-  __shared__ float swt[H * TP];
-  __shared__ float swr[H * TP];
+  __shared__ __attribute__((aligned(16))) float swt[H * H];
+  __shared__ __attribute__((aligned(16))) float swr[H * H];
   __shared__ __attribute__((aligned(16))) float strans[2 * O::TSZ];
   __shared__ float ssmall[4 * NS * H];
   __shared__ float sbl[64];
