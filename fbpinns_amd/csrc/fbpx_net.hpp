@@ -293,6 +293,31 @@ __device__ inline LaneLds lane_lds(const float* ss, const float* szt, const floa
   return v;
 }
 
+// Z[ob] += sum over K-steps q of the fragment A operands (LDS, lane-major,
+// K-step q at frag + q*64*NRB) times B[q >> 2][q & 3] (registers).  A
+// scheduling region of its own in which each fragment is read two K-steps
+// ahead of the MFMAs that use it (group barriers: two reads, then per
+// K-step its NRB MFMAs and the next read), so the LDS latency is hidden
+// behind MFMAs instead of stalling each K-step.
+template <int H, class BOp>
+__device__ inline void frag_pass(const float* frag, BOp b, f4* Z) {
+  constexpr int NRB = H / 16, NQ = 4 * NRB;
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const Frag<NRB> w = ld_frag<NRB>(frag + q * 64 * NRB);
+#pragma unroll
+    for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(w.v[ob], b[q >> 2][q & 3], Z[ob]);
+  }
+  __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    __builtin_amdgcn_sched_group_barrier(0x008, NRB, 0);
+    if (q + 2 < NQ) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // Layer 1 for this lane's units (rows 16rb + 4g + r): h = tanh(W1 xh + b1).
 template <int XD, int H, class SM>
 __device__ inline void layer1_stream(const LaneLds& v, const float* xh, float (*h1)[4]) {
@@ -315,6 +340,20 @@ __device__ inline void layer1_stream(const LaneLds& v, const float* xh, float (*
 template <int H, class SM>
 __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int zrow, int kind, f4* Z) {
   constexpr int NRB = H / 16;
+  float bv[NRB][4];
+#pragma unroll
+  for (int kb = 0; kb < NRB; ++kb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const float h = h1[kb][ks];
+      float b = h;
+      if (kind == 1) {
+        b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];            // d1 zt
+      } else if (kind == 2) {
+        b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];      // d2 zt^2 = h d1 (-2 zt^2)
+      }
+      bv[kb][ks] = b;
+    }
 #pragma unroll
   for (int ob = 0; ob < NRB; ++ob) {
     if (kind == 0) {
@@ -324,22 +363,7 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
       Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
     }
   }
-#pragma unroll
-  for (int kb = 0; kb < NRB; ++kb) {
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const Frag<NRB> w = ld_frag<NRB>(v.wt + (4 * kb + ks) * 64 * NRB);
-      const float h = h1[kb][ks];
-      float b = h;
-      if (kind == 1) {
-        b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];            // d1 zt
-      } else if (kind == 2) {
-        b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];      // d2 zt^2 = h d1 (-2 zt^2)
-      }
-#pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(w.v[ob], b, Z[ob]);
-    }
-  }
+  frag_pass<H>(v.wt, bv, Z);
 }
 
 // The lane's A-operand fragments of W2 for every layer-2 MFMA, loaded once
@@ -395,17 +419,9 @@ __device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H /
 // Z[ob] = W2 . b, b[kb][ks] the value for unit 16kb + 4g + ks of this lane's pair.
 template <int H>
 __device__ inline void layer2_pass_raw(const LaneLds& v, const float (*b)[4], f4* Z) {
-  constexpr int NRB = H / 16;
 #pragma unroll
-  for (int ob = 0; ob < NRB; ++ob) Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kb = 0; kb < NRB; ++kb)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const Frag<NRB> w = ld_frag<NRB>(v.wt + (4 * kb + ks) * 64 * NRB);
-#pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(w.v[ob], b[kb][ks], Z[ob]);
-    }
+  for (int ob = 0; ob < H / 16; ++ob) Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
+  frag_pass<H>(v.wt, b, Z);
 }
 
 // The pair's layer-1 combination of every tangent the upstream gradient
@@ -549,14 +565,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
         f4 vt[NRB];
 #pragma unroll
         for (int ib = 0; ib < NRB; ++ib) vt[ib] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ob = 0; ob < NRB; ++ob)
-#pragma unroll
-          for (int ks = 0; ks < 4; ++ks) {
-            const Frag<NRB> w = ld_frag<NRB>(v.wr + (4 * ob + ks) * 64 * NRB);
-#pragma unroll
-            for (int ib = 0; ib < NRB; ++ib) vt[ib] = mfma16(w.v[ib], wd1[ob][ks], vt[ib]);
-          }
+        frag_pass<H>(v.wr, wd1, vt);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
@@ -798,17 +807,9 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
 // from the padded row-major copy wr; B operand = G in D layout).
 template <int H>
 __device__ inline void w2t_pass(const LaneLds& v, const f4* G, f4* GA) {
-  constexpr int NRB = H / 16;
 #pragma unroll
-  for (int ib = 0; ib < NRB; ++ib) GA[ib] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int ob = 0; ob < NRB; ++ob)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const Frag<NRB> w = ld_frag<NRB>(v.wr + (4 * ob + ks) * 64 * NRB);
-#pragma unroll
-      for (int ib = 0; ib < NRB; ++ib) GA[ib] = mfma16(w.v[ib], G[ob][ks], GA[ib]);
-    }
+  for (int ib = 0; ib < H / 16; ++ib) GA[ib] = f4{0.f, 0.f, 0.f, 0.f};
+  frag_pass<H>(v.wr, G, GA);
 }
 
 // Workgroup-shared dW2 accumulation.  The 4 waves of a workgroup process
@@ -949,12 +950,6 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     float vWL[NRB][4], gz10[NRB][4], vv[NRB][4];
 
     __syncthreads();   // the previous tile's transposes have been consumed
-    {
-      f4 hv[NRB];
-#pragma unroll
-      for (int rb = 0; rb < NRB; ++rb) hv[rb] = f4{h1[rb][0], h1[rb][1], h1[rb][2], h1[rb][3]};
-      store_tile<H>(stbw, hv);
-    }
 
     // ---- Z_0 = W2 h1 + b2 -> h2
     float h2[NRB][4];
@@ -968,14 +963,14 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __builtin_amdgcn_sched_barrier(0);
 
-    // ---- S = W2 s,  G_S = f1,  vt = W2~^T G_S (d L / d s).  With
+    // ---- S = W2 s,  G_S = f1,  vt = W2~^T G_S (d L / d s),  dW2 += G_S s^T
+    //      (column S first, so that s is dead before the axes).  With
     //      Q = sum_i gur_ii Z_i^2 (accumulated per axis):
     //        G_0 = f1 gur_0 + f2 S + f3 Q,  dWL = gur_0 h2 + f1 S + f2 Q
     //      (f1 = 1 - h2^2, f2 = -2 h2 f1, f3 = f1 (4 - 6 f1)).  The layer-1
     //      gradient starts as gz1_0 = vt rv (see pair_s); the axes add
     //      e2 zt_i GA_i and column 0 adds e1 GA_0.
     f4 G0[NRB], vt[NRB];
-    float sv[NRB][4];   // s, kept for the column-S phase
     float Sx[NRB][4], Q[NRB][4];
     if constexpr (S::N1 == 0) {
       // value only: no tangents, s = 0
@@ -989,7 +984,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           gz10[ob][r] = 0.f;
         }
     } else {
-      float rv[NRB][4];
+      float sv[NRB][4], rv[NRB][4];
       pair_s<S, H>(v, h1, gur, sv, rv);
       {
         f4 Sv[NRB];
@@ -1003,15 +998,33 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           }
       }
       f4 gs[NRB];
+      {
+        f4 sw[NRB];
 #pragma unroll
-      for (int ob = 0; ob < NRB; ++ob)
+        for (int ob = 0; ob < NRB; ++ob) {
+          sw[ob] = f4{sv[ob][0], sv[ob][1], sv[ob][2], sv[ob][3]};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) gs[ob][r] = fmaf(-h2[ob][r], h2[ob][r], 1.f);
+          for (int r = 0; r < 4; ++r) gs[ob][r] = fmaf(-h2[ob][r], h2[ob][r], 1.f);
+        }
+        store_tile<H>(stbw, sw);
+        store_tile<H>(stgw, gs);
+      }
       w2t_pass<H>(v, gs, vt);
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) gz10[rb][r] = vt[rb][r] * rv[rb][r];
+      __syncthreads();
+      dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
+      __syncthreads();
+    }
+    // layer-1 h for the B operands of the remaining dW2 columns (published
+    // with the first column's G; every column has a barrier before its reads)
+    {
+      f4 hv[NRB];
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) hv[rb] = f4{h1[rb][0], h1[rb][1], h1[rb][2], h1[rb][3]};
+      store_tile<H>(stbw, hv);
     }
     __builtin_amdgcn_sched_barrier(0);
 
@@ -1112,22 +1125,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __syncthreads();
     dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
-    __syncthreads();
-    // ---- column S: dW2 += G_S s^T (G_S = f1 rebuilt from h2)
-    if constexpr (S::N1 > 0) {
-      f4 sw[NRB], gs[NRB];
-#pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) {
-        sw[ob] = f4{sv[ob][0], sv[ob][1], sv[ob][2], sv[ob][3]};
-#pragma unroll
-        for (int r = 0; r < 4; ++r) gs[ob][r] = fmaf(-h2[ob][r], h2[ob][r], 1.f);
-      }
-      store_tile<H>(stbw, sw);
-      store_tile<H>(stgw, gs);
-      __syncthreads();
-      dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
-      // (the next tile's leading barrier orders these reads before new writes)
-    }
+    // (the next tile's leading barrier orders these reads before new writes)
 
     // dW1[:, i] += gz1_0 xh_i (the gz1_i parts were added per axis)
 #pragma unroll
