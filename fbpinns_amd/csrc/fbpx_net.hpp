@@ -336,6 +336,7 @@ __device__ inline void layer1_stream(const LaneLds& v, const float* xh, float (*
 
 // One layer-2 MFMA pass: Z[ob] (+)= W2 . B with B[kb][ks] = base * scale.
 // kind 0: B = h (Z initialised with b2), 1: B = d1 zt_i, 2: B = d2 zt_i^2,
+// 3: B = base zt_i with base = d1 supplied in place of h;
 // zrow = i * H selects the zt row of axis i.
 template <int H, class SM>
 __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int zrow, int kind, f4* Z) {
@@ -351,6 +352,8 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
         b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];            // d1 zt
       } else if (kind == 2) {
         b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];      // d2 zt^2 = h d1 (-2 zt^2)
+      } else if (kind == 3) {
+        b = h * v.zt[zrow + 16 * kb + ks];                           // (given d1) zt
       }
       bv[kb][ks] = b;
     }
@@ -1026,6 +1029,17 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       for (int rb = 0; rb < NRB; ++rb) hv[rb] = f4{h1[rb][0], h1[rb][1], h1[rb][2], h1[rb][3]};
       store_tile<H>(stbw, hv);
     }
+    // from here on layer 1 enters only through e1 = 1 - h1^2 and h1 e1, and
+    // the axes take layer 2 through p2 = h2 - h2^3 = -f2 / 2
+    float e1[NRB][4], he[NRB][4], p2[NRB][4];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        e1[rb][r] = fmaf(-h1[rb][r], h1[rb][r], 1.f);
+        he[rb][r] = h1[rb][r] * e1[rb][r];
+        p2[rb][r] = fmaf(-h2[rb][r] * h2[rb][r], h2[rb][r], h2[rb][r]);
+      }
     __builtin_amdgcn_sched_barrier(0);
 
     // ---- per axis.  d L / d a1_i = gur_i vt (+ W2~^T G_i when the axis has a
@@ -1042,24 +1056,20 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       const float gur_b = two ? pick<J>(gur, S::col2(i)) : 0.f;
       const float isd_i = pick<XD>(isd, i);
       const int zrow = i * H;
-      opaque<NRB>(h1);
-      opaque<NRB>(h2);
       float gzi[NRB][4];
       if (two) {
         f4 Gi[NRB];
         {
           f4 Zi[NRB];
-          layer2_pass<H, SM>(v, h1, zrow, 1, Zi);
+          layer2_pass<H, SM>(v, e1, zrow, 3, Zi);
           const float gm4 = -4.f * gur_b;
 #pragma unroll
           for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const float h = h2[ob][r];
               const float z = Zi[ob][r];
               Q[ob][r] = fmaf(gur_b * z, z, Q[ob][r]);
-              // 2 gur_b f2 z = -4 gur_b (h - h^3) z
-              Gi[ob][r] = gm4 * (fmaf(-h * h, h, h) * z);
+              Gi[ob][r] = gm4 * (p2[ob][r] * z);   // 2 gur_b f2 z
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1073,11 +1083,9 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const float h = h1[rb][r];
-              const float e1 = fmaf(-h, h, 1.f);
-              const float p = (h * e1) * v.zm[zrow + 16 * rb + r];   // e2 zt_i
-              const float q = fmaf(gb2, p, gur_a * e1);
-              gzi[rb][r] = fmaf(vt[rb][r], q, e1 * GA[rb][r]);
+              const float p = he[rb][r] * v.zm[zrow + 16 * rb + r];   // e2 zt_i
+              const float q = fmaf(gb2, p, gur_a * e1[rb][r]);
+              gzi[rb][r] = fmaf(vt[rb][r], q, e1[rb][r] * GA[rb][r]);
               gz10[rb][r] = fmaf(p, GA[rb][r], gz10[rb][r]);
             }
         }
@@ -1088,7 +1096,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) gzi[rb][r] = fmaf(-h1[rb][r], h1[rb][r], 1.f) * (gur_a * vt[rb][r]);
+          for (int r = 0; r < 4; ++r) gzi[rb][r] = e1[rb][r] * (gur_a * vt[rb][r]);
       }
       // dW1[:, i] through z1_i = zt_i (the 1 / sd_i factor after the sum)
       reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
@@ -1101,14 +1109,13 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         for (int r = 0; r < 4; ++r) {
           const float h = h2[ob][r];
           const float f1 = fmaf(-h, h, 1.f);
-          const float f2 = -2.f * (h * f1);
+          const float f2 = -2.f * p2[ob][r];
           const float f3 = f1 * fmaf(-6.f, f1, 4.f);
           G0[ob][r] = fmaf(f3, Q[ob][r], fmaf(f2, Sx[ob][r], f1 * gur[0]));
           vWL[ob][r] = fmaf(f2, Q[ob][r], fmaf(f1, Sx[ob][r], gur[0] * h));
         }
     }
     // ---- column 0: db2 += G_0, dW2 += G_0 h^T, W2~^T G_0 -> layer 1
-    opaque<NRB>(h1);
 #pragma unroll
     for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
@@ -1121,7 +1128,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(fmaf(-h1[rb][r], h1[rb][r], 1.f), GA[rb][r], gz10[rb][r]);
+        for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(e1[rb][r], GA[rb][r], gz10[rb][r]);
     }
     __syncthreads();
     dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
