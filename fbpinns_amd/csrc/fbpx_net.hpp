@@ -930,22 +930,35 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   // every wave runs the same number of tiles (barriers below); lanes past the
   // chunk end carry zero upstream gradient and so contribute nothing
   const int ntile = (end - beg + 63) >> 6;
-  for (int t = 0; t < ntile; ++t) {
-    const int k = beg + t * 64 + wave * 16 + pl;
-    const bool valid = k < end;
-    const int kk = valid ? k : beg;
+  // each tile's point and upstream jets are fetched during the previous tile
+  // (indices at the start of its column-0 phase, data before its last dW2
+  // column), so the two dependent global loads do not stall the tile start
+  float xn[XD], gUn[J];
+  {
+    const int k = beg + wave * 16 + pl;
+    const int kk = k < end ? k : beg;
     const int p = wl.pair_point[kk];
     const int seg = wl.pair_seg[kk];
+#pragma unroll
+    for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)p * XD + i];
+#pragma unroll
+    for (int c = 0; c < J; ++c) gUn[c] = k < end ? gseg[(size_t)seg * J + c] : 0.f;
+  }
+  for (int t = 0; t < ntile; ++t) {
     float xv[XD], xh[XD];
 #pragma unroll
     for (int i = 0; i < XD; ++i) {
-      xv[i] = x[(size_t)p * XD + i];
+      xv[i] = xn[i];
       xh[i] = (xv[i] - mu[i]) * isd[i];
     }
-    float gur[J], gU[J];
+    float gur[J];
+    pair_upstream<S>(gUn, xv, d, gur);
+    // materialise the pair's values here, so that the prefetched data dies
+    // at the tile start instead of being carried into the axis phases
 #pragma unroll
-    for (int c = 0; c < J; ++c) gU[c] = valid ? gseg[(size_t)seg * J + c] : 0.f;
-    pair_upstream<S>(gU, xv, d, gur);
+    for (int c = 0; c < J; ++c) asm volatile("" : "+v"(gur[c]));
+#pragma unroll
+    for (int i = 0; i < XD; ++i) asm volatile("" : "+v"(xh[i]));
     if (g == 0) dbl += gur[0];
 
     float h1[NRB][4];
@@ -1116,6 +1129,10 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         }
     }
     // ---- column 0: db2 += G_0, dW2 += G_0 h^T, W2~^T G_0 -> layer 1
+    const int kn = beg + (t + 1) * 64 + wave * 16 + pl;   // next tile's pair
+    const bool valid_n = kn < end;
+    const int pn = wl.pair_point[valid_n ? kn : beg];
+    const int segn = wl.pair_seg[valid_n ? kn : beg];
 #pragma unroll
     for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
@@ -1130,10 +1147,6 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
         for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(e1[rb][r], GA[rb][r], gz10[rb][r]);
     }
-    __syncthreads();
-    dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
-    // (the next tile's leading barrier orders these reads before new writes)
-
     // dW1[:, i] += gz1_0 xh_i (the gz1_i parts were added per axis)
 #pragma unroll
     for (int i = 0; i < XD; ++i) {
@@ -1145,6 +1158,15 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     reduce_units_to_lds<NRB>(vWL, mysmall + K_WL * H, pl, g);
     reduce_units_to_lds<NRB>(gz10, mysmall + K_B1 * H, pl, g);
+    // the next tile's point and upstream jets, in flight during the last dW2 column
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
+#pragma unroll
+    for (int c = 0; c < J; ++c) gUn[c] = valid_n ? gseg[(size_t)segn * J + c] : 0.f;
+    __syncthreads();
+    dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
+    // (the next tile's leading barrier orders these reads before new writes)
   }
 
   // ---- end of chunk: the transpose buffers become the gradient row
