@@ -20,7 +20,7 @@ from fbpinns_amd import ops
 from fbpinns_amd._lib import FBPX_CHUNK
 from fbpinns_amd.util.logger import logger
 
-CANDIDATES = (256, 512, 1024, 2048)
+CANDIDATES = (128, 256, 512, 1024, 2048)
 MIN_PAIRS = 200_000
 _cache = {}
 

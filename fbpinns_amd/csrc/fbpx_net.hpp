@@ -325,10 +325,9 @@ __device__ inline void layer1_stream(const LaneLds& v, const float* xh, float (*
   for (int rb = 0; rb < H / 16; ++rb) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      float z = 0.f;
+      float z = v.vec[SM::B1 + 16 * rb + r];
 #pragma unroll
       for (int i = 0; i < XD; ++i) z = fmaf(v.w1[(16 * rb + r) * XD + i], xh[i], z);
-      z += v.vec[SM::B1 + 16 * rb + r];
       h1[rb][r] = tanh_f(z);
     }
   }
@@ -1175,8 +1174,10 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   float* sacc = strans;
   for (int i = threadIdx.x; i < L::P; i += blockDim.x) sacc[i] = 0.f;
   __syncthreads();
-  for (int w = 0; w < nw; ++w) {
-    if (wave == w) {
+  // H >= 32: the waves hold disjoint dW2 blocks and write them at once;
+  // H = 16: every wave holds the one block, summed wave by wave
+  for (int w = 0; w < (O::NSRC == 1 ? nw : 1); ++w) {
+    if (O::NSRC != 1 || wave == w) {
 #pragma unroll
       for (int j = 0; j < O::NOB; ++j) {
         const int ob = O::row_block(wave, j);
