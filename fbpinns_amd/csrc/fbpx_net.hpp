@@ -108,10 +108,21 @@ __device__ inline Frag<NRB> ld_frag(const float* p) {
   return f;
 }
 
+// Sum over the four 16-lane rows (lane groups g) for every pl, in registers:
+// v_permlane32_swap exchanges the upper half of its first operand with the
+// lower half of its second, so with both holding v the two results add to
+// v[l] + v[l ^ 32]; v_permlane16_swap does the same between odd and even
+// rows.  Written as asm: the compiler builtins for these instructions read
+// one result register twice on this toolchain.  The s_nop covers the
+// VALU-write -> permlane-read hazard.
 __device__ inline float sum_groups(float v) {
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
+  float a = v, b = v;
+  asm("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  float s = a + b;
+  a = s;
+  b = s;
+  asm("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+  return a + b;
 }
 
 template <int CTRL>
