@@ -379,55 +379,6 @@ __device__ inline void layer2_pass(const LaneLds& v, const float (*h1)[4], int z
   frag_pass<H>(v.wt, bv, Z);
 }
 
-// The lane's A-operand fragments of W2 for every layer-2 MFMA, loaded once
-// per chunk: wf[kb][ks][ob] = W2[16ob + pl][16kb + 4g + ks] (H * H / 64
-// registers), so the forward passes read no weights from LDS.
-template <int H>
-__device__ inline void load_w2_frag(const LaneLds& v, float (*wf)[4][H / 16]) {
-#pragma unroll
-  for (int kb = 0; kb < H / 16; ++kb)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const Frag<H / 16> w = ld_frag<H / 16>(v.wt + (4 * kb + ks) * 64 * (H / 16));
-#pragma unroll
-      for (int ob = 0; ob < H / 16; ++ob) wf[kb][ks][ob] = w.v[ob];
-    }
-}
-
-// layer2_pass with the weights taken from register fragments; kind 3:
-// B = base zt_i with base = d1 of layer 1 supplied in place of h.
-template <int H, class SM>
-__device__ inline void layer2_pass_rf(const LaneLds& v, const float (*wf)[4][H / 16], const float (*h1)[4], int zrow,
-                                      int kind, f4* Z) {
-  constexpr int NRB = H / 16;
-#pragma unroll
-  for (int ob = 0; ob < NRB; ++ob) {
-    if (kind == 0) {
-      const float* b2 = v.vec + SM::B2 + 16 * ob;
-      Z[ob] = f4{b2[0], b2[1], b2[2], b2[3]};
-    } else {
-      Z[ob] = f4{0.f, 0.f, 0.f, 0.f};
-    }
-  }
-#pragma unroll
-  for (int kb = 0; kb < NRB; ++kb) {
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const float h = h1[kb][ks];
-      float b = h;
-      if (kind == 1) {
-        b = fmaf(-h, h, 1.f) * v.zt[zrow + 16 * kb + ks];
-      } else if (kind == 2) {
-        b = (h * fmaf(-h, h, 1.f)) * v.zq[zrow + 16 * kb + ks];
-      } else if (kind == 3) {
-        b = h * v.zt[zrow + 16 * kb + ks];                           // (given d1) zt
-      }
-#pragma unroll
-      for (int ob = 0; ob < NRB; ++ob) Z[ob] = mfma16(wf[kb][ks][ob], b, Z[ob]);
-    }
-  }
-}
-
 // One layer-2 MFMA pass with the B operand given in registers (no bias):
 // Z[ob] = W2 . b, b[kb][ks] the value for unit 16kb + 4g + ks of this lane's pair.
 template <int H>
@@ -466,7 +417,7 @@ __device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const floa
 // ------------------------------------------------------------------ forward
 
 template <int XD, int H, int NH, int D1M, int D2M>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, NH == 2 ? 3 : 2)
 fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, float* __restrict__ pairbuf) {
   using S = Schema<XD, D1M, D2M>;
@@ -500,8 +451,6 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int pl = lane & 15, g = lane >> 4;
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
-  float wf[NH == 2 ? NRB : 1][4][NH == 2 ? NRB : 1];
-  if constexpr (NH == 2) load_w2_frag<H>(v, wf);
   // the point of each sub-tile is fetched one sub-tile ahead (pair index at
   // the top of the previous sub-tile, coordinates in its middle), so the two
   // dependent global loads overlap the previous sub-tile's compute
@@ -548,7 +497,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
         }
     } else {
       f4 Z0[NRB];
-      layer2_pass_rf<H, SM>(v, wf, h1, 0, 0, Z0);
+      layer2_pass<H, SM>(v, h1, 0, 0, Z0);
       // output-layer weights times the layer-2 tanh derivatives, per element:
       // wd1 = WL f1 and wd2 = WL f1 h2 = WL f2 / (-2), so that u_i = sum wd1 Z_i
       // and the Z_i^2 part of u_ii = sum WL (f1 Z_ii + f2 Z_i^2) costs two FMAs;
@@ -602,7 +551,7 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
           continue;
         }
         f4 Zi[NRB];
-        layer2_pass_rf<H, SM>(v, wf, e1, i * H, 3, Zi);
+        layer2_pass<H, SM>(v, e1, i * H, 3, Zi);
 #pragma unroll
         for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
