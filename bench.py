@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true", help="launch the step eagerly (for counter passes)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default chunk size (counter passes see one launch shape)")
+                    help="keep the default launch shape (counter passes see one launch shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
     a = ap.parse_args()
@@ -203,7 +203,7 @@ def main():
         calibrate_copy(a.calibrate_copy)
         return
 
-    from fbpinns_amd import configs
+    from fbpinns_amd import autotune, configs
     from fbpinns_amd.plain import PlainFBPINN
     from fbpinns_amd.trainers import FBPINNTrainer
     from fbpinns_amd.util.logger import logger
@@ -284,7 +284,9 @@ def main():
         "config": {"workload": f"{a.config}: {n_pts} collocation points, {eng.m} subdomains, "
                                f"{eng.plan.wl.n_pairs} (point, subdomain) pairs, FCN [{eng.net.xd}, "
                                f"{eng.net.h}x{eng.net.nh}, 1], jets J={eng.net.J}, hipGraph-captured step",
-                   "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs},
+                   "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs,
+                   "launch": {"forward": f"{autotune.name(eng.launch[0])}, {eng.plan.wl.n_fbins} workgroups",
+                              "backward": f"{autotune.name(eng.launch[1])}, {eng.plan.wl.n_bbins} workgroups"}},
         "collocation_pts_per_s": sps * n_pts,
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP32_MFMA, "traffic": traffic,

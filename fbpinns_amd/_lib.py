@@ -31,8 +31,10 @@ class fbpx_net(ctypes.Structure):
 class fbpx_worklist(ctypes.Structure):
     _fields_ = [
         ("n_pairs", _c_i32), ("n_chunks", _c_i32), ("n_points", _c_i32), ("n_segs", _c_i32), ("npou", _c_i32),
+        ("n_fbins", _c_i32), ("n_bbins", _c_i32),
         ("pair_point", _c_p), ("pair_seg", _c_p), ("chunk_model", _c_p), ("chunk_start", _c_p),
         ("point_seg_start", _c_p), ("seg_pair_start", _c_p), ("pair_slot", _c_p),
+        ("fbin_chunk_start", _c_p), ("bbin_chunk_start", _c_p),
     ]
 
 
@@ -47,6 +49,7 @@ SIGNATURES = {
     "fbpx_pou_reduce": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p]),
     "fbpx_pou_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p,
                                    _c_p]),
+    "fbpx_occupancy": (_c_i32, [ctypes.POINTER(fbpx_net), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32)]),
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                _c_p, _c_p, _c_p]),
     "fbpx_reduce_rows": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p]),

@@ -47,13 +47,17 @@ static int check_pair_args(const fbpx_net* net, const float* theta, const float*
                            const fbpx_worklist* wl, const void* out) {
   int rc = check_net(net);
   if (rc) return rc;
-  if (!wl || wl->n_pairs < 0 || wl->n_chunks < 0) {
+  if (!wl || wl->n_pairs < 0 || wl->n_chunks < 0 || wl->n_fbins < 0 || wl->n_bbins < 0) {
     set_error("bad work list");
     return FBPX_EINVAL;
   }
   if (wl->n_chunks > 0 && (!theta || !dom || !x || !out || !wl->chunk_model || !wl->chunk_start ||
                            !wl->pair_point)) {
     set_error("null pointer in pair kernel arguments");
+    return FBPX_EINVAL;
+  }
+  if (wl->n_chunks > 0 && (wl->n_fbins < 1 || wl->n_bbins < 1 || !wl->fbin_chunk_start || !wl->bbin_chunk_start)) {
+    set_error("work list has chunks but no bins (n_fbins=%d n_bbins=%d)", wl->n_fbins, wl->n_bbins);
     return FBPX_EINVAL;
   }
   return FBPX_OK;
@@ -95,6 +99,25 @@ int fbpx_backward(void* stream, const fbpx_net* net, const float* theta, const f
   static const launch_fn fns[3] = {launch_bwd_xd1, launch_bwd_xd2, launch_bwd_xd3};
   NetArgs a{theta, dom, x, *wl, chunk_train, gseg, partial};
   return dispatch(fns, net, a, (hipStream_t)stream);
+}
+
+int fbpx_occupancy(const fbpx_net* net, int32_t* fwd_per_cu, int32_t* bwd_per_cu) {
+  int rc = check_net(net);
+  if (rc) return rc;
+  if (!fwd_per_cu || !bwd_per_cu) {
+    set_error("occupancy: null output");
+    return FBPX_EINVAL;
+  }
+  typedef int (*occ_fn)(const fbpx_net&, int*, int*);
+  static const occ_fn fns[3] = {occupancy_xd1, occupancy_xd2, occupancy_xd3};
+  int f = 0, b = 0;
+  rc = fns[net->xd - 1](*net, &f, &b);
+  if (rc == FBPX_EUNSUPPORTED)
+    set_error("no kernel for xd=%d h=%d nh=%d d1mask=%d d2mask=%d", net->xd, net->h, net->nh, net->d1mask,
+              net->d2mask);
+  *fwd_per_cu = f;
+  *bwd_per_cu = b;
+  return rc;
 }
 
 int32_t fbpx_param_count(const fbpx_net* net) {

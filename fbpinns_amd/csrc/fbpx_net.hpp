@@ -416,10 +416,11 @@ __device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const floa
 
 // ------------------------------------------------------------------ forward
 
+// One chunk of the forward (a run of pairs of one subdomain).
 template <int XD, int H, int NH, int D1M, int D2M>
-__global__ void __launch_bounds__(256, NH == 2 ? 3 : 2)
-fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
-           fbpx_worklist wl, float* __restrict__ pairbuf) {
+__device__ inline void fwd_chunk(const float* __restrict__ theta, const float* __restrict__ dom,
+                                 const float* __restrict__ x, const fbpx_worklist& wl, float* __restrict__ pairbuf,
+                                 int chunk) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, NH>;
   using SM = Small<XD, H, NH>;
@@ -429,7 +430,6 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   __shared__ __attribute__((aligned(16))) float swt[NH == 2 ? H * H : 1];
   __shared__ __attribute__((aligned(16))) float swr[NH == 2 && S::N1 > 0 ? H * H : 1];
 
-  const int chunk = blockIdx.x;
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
   const int end = wl.chunk_start[chunk + 1];
@@ -582,6 +582,19 @@ fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
       for (int c = 0; c < J; ++c)
         if ((c & 3) == g) pairbuf[(size_t)slot * J + c] = uw[c];
     }
+  }
+}
+
+// Persistent forward launch: workgroup b runs the chunks of forward bin b
+// in order.
+template <int XD, int H, int NH, int D1M, int D2M>
+__global__ void __launch_bounds__(256, NH == 2 ? 3 : 2)
+fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
+           fbpx_worklist wl, float* __restrict__ pairbuf) {
+  const int c0 = wl.fbin_chunk_start[blockIdx.x], c1 = wl.fbin_chunk_start[blockIdx.x + 1];
+  for (int chunk = c0; chunk < c1; ++chunk) {
+    if (chunk > c0) __syncthreads();   // every wave is done with the previous chunk's LDS
+    fwd_chunk<XD, H, NH, D1M, D2M>(theta, dom, x, wl, pairbuf, chunk);
   }
 }
 
@@ -1156,12 +1169,16 @@ __global__ void __launch_bounds__(256, 2)
 bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, const uint8_t* __restrict__ chunk_train, const float* __restrict__ gseg,
            float* __restrict__ partial) {
-  const int chunk = blockIdx.x;
-  if (chunk_train != nullptr && chunk_train[chunk] == 0) return;
-  if constexpr (NH == 1)
-    bwd_one<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
-  else
-    bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
+  // persistent: workgroup b runs the chunks of backward bin b in order
+  const int c0 = wl.bbin_chunk_start[blockIdx.x], c1 = wl.bbin_chunk_start[blockIdx.x + 1];
+  for (int chunk = c0; chunk < c1; ++chunk) {
+    if (chunk_train != nullptr && chunk_train[chunk] == 0) continue;
+    __syncthreads();   // every wave is done with the previous chunk's LDS
+    if constexpr (NH == 1)
+      bwd_one<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
+    else
+      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
+  }
 }
 
 }  // namespace fbpx

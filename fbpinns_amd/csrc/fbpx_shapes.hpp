@@ -37,6 +37,10 @@ int launch_fwd_xd3(const fbpx_net& n, const NetArgs& a, hipStream_t s);
 int launch_bwd_xd1(const fbpx_net& n, const NetArgs& a, hipStream_t s);
 int launch_bwd_xd2(const fbpx_net& n, const NetArgs& a, hipStream_t s);
 int launch_bwd_xd3(const fbpx_net& n, const NetArgs& a, hipStream_t s);
+// Resident 256-thread workgroups per CU of the forward and backward kernels.
+int occupancy_xd1(const fbpx_net& n, int* fwd, int* bwd);
+int occupancy_xd2(const fbpx_net& n, int* fwd, int* bwd);
+int occupancy_xd3(const fbpx_net& n, int* fwd, int* bwd);
 bool schema_supported(int xd, int d1, int d2);
 
 }  // namespace fbpx

@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from fbpinns_amd import autotune, ops
-from fbpinns_amd._lib import FBPX_CHUNK, FBPX_DOM_STRIDE, FBPX_MAX_XD
+from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
 from fbpinns_amd.epilogue import Epilogue, schema_masks
 from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
 from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch
@@ -190,10 +190,12 @@ class FusedEngine:
         if not self.net.supported():
             raise FbpxError(f"no kernels for {self.net}")
 
-    def set_active(self, active, chunk=None):
+    def set_active(self, active, chunk=None, launch=None):
         """New scheduler state: rebuild takes, work list, epilogue and graph
-        (trainers.py:509-575 and the recompile at :647-653).  chunk: pairs
-        per workgroup (default: autotuned, or FBPX_CHUNK with autotune off)."""
+        (trainers.py:509-575 and the recompile at :647-653).  launch:
+        (forward, backward) launch shapes, each ("chunks", C) or ("bins", k)
+        (see autotune); chunk=C is (("chunks", C), ("chunks", C)).  Default:
+        autotuned, or autotune.default_spec with autotune off."""
         active = np.asarray(active).astype(np.int64)
         training_ips, cfs, cips, dstat = get_x_batch(active, self.all_params, self.x_global, self.c_offsets,
                                                      self.c_sizes, self.decomposition)
@@ -210,15 +212,30 @@ class FusedEngine:
         takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
         n_active = int((active_ == 1).sum())
 
-        def build(chunk):
-            return build_worklist(takes, all_ims, n_active, x.shape[0], self.device, chunk=chunk)
+        n_pairs = int(len(takes[0]))
+        slots = []
+
+        def build(fspec, bspec):
+            caps = [c[1] for c in (fspec, bspec) if c[0] == "chunks"]
+            fb = bb = None
+            if "bins" in (fspec[0], bspec[0]):
+                if not slots:   # resident workgroups of each kernel on the whole GPU
+                    fo, bo = self.net.occupancy()
+                    n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count
+                    slots.extend([max(1, fo) * n_cu, max(1, bo) * n_cu])
+                fb = fspec[1] * slots[0] if fspec[0] == "bins" else None
+                bb = bspec[1] * slots[1] if bspec[0] == "bins" else None
+            return build_worklist(takes, all_ims, n_active, x.shape[0], self.device,
+                                  chunk=min(caps) if caps else None, fbins=fb, bbins=bb)
         if chunk is not None:
-            self.chunk, plan = int(chunk), build(int(chunk))
+            launch = (("chunks", int(chunk)), ("chunks", int(chunk)))
+        if launch is not None:
+            self.launch, plan = (tuple(launch[0]), tuple(launch[1])), build(tuple(launch[0]), tuple(launch[1]))
         elif self.autotune_enabled:
-            self.chunk, plan = autotune.tune_chunk(build, self.net, self.theta, self.dom, x, self.m,
-                                                   int(len(takes[0])))
+            self.launch, plan = autotune.tune_plan(build, self.net, self.theta, self.dom, x, self.m, n_pairs)
         else:
-            self.chunk, plan = FBPX_CHUNK, build(FBPX_CHUNK)
+            spec = autotune.default_spec(n_pairs)
+            self.launch, plan = (spec, spec), build(spec, spec)
         wsum = torch.empty((max(plan.wl.n_segs, 1), self.net.J), dtype=torch.float32, device=self.device)
         scratch = torch.empty((max(plan.wl.n_pairs, 1), self.net.J), dtype=torch.float32, device=self.device)
         ops.pou_prepare(self.net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)
@@ -231,8 +248,9 @@ class FusedEngine:
         self.takes = takes
         self.epilogue = Epilogue(self.problem, self.all_params, x, cranges, cvalues, self.required_ujss, self.net)
         self.n_active = n_active
-        logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks "
-                    f"of <= {self.chunk}, "
+        logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks, "
+                    f"fwd {autotune.name(self.launch[0])} ({plan.wl.n_fbins} workgroups), "
+                    f"bwd {autotune.name(self.launch[1])} ({plan.wl.n_bbins} workgroups), "
                     f"{n_active} trained / {len(all_ims)} used subdomains, jets J={self.net.J}, "
                     f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}, d={dstat:.2f}")
         self._graph = None

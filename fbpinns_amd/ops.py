@@ -57,6 +57,13 @@ class NetSpec:
     def supported(self):
         return _lib.supported(*self.as_list())
 
+    def occupancy(self):
+        """(forward, backward) resident 256-thread workgroups per CU of the
+        pair kernels for this shape (fbpx_occupancy; needs the GPU)."""
+        f, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().fbpx_occupancy(ctypes.byref(self.c()), ctypes.byref(f), ctypes.byref(b)), "fbpx_occupancy")
+        return int(f.value), int(b.value)
+
     def col1(self, i):
         return 1 + bin(self.d1mask & ((1 << i) - 1)).count("1")
 
@@ -74,8 +81,8 @@ def _net_from_list(net: List[int]):
 # ------------------------------------------------------------------ work list
 
 _WL_TENSORS = ["pair_point", "pair_seg", "chunk_model", "chunk_start", "point_seg_start", "seg_pair_start",
-               "pair_slot"]
-_WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou"]
+               "pair_slot", "fbin_chunk_start", "bbin_chunk_start"]
+_WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou", "n_fbins", "n_bbins"]
 
 
 class Worklist:

@@ -144,30 +144,56 @@ class WorkPlan:
         self.n_active, self.all_ims = n_active, all_ims
 
 
-def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK):
+def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK, fbins=None, bbins=None):
     """Kernel work list from the reference-format takes.
 
     Tile order groups the pairs by position in all_ims = [active; fixed]
-    (stable, so ascending point order inside a subdomain); each subdomain's
-    run is cut into chunks of <= `chunk` pairs.  Segments are the unique
-    (point, pou) rows; pair_slot gives each tile-order pair its slot in
-    segment order (a segment's pairs are contiguous there, in the reference's
-    pair order)."""
+    (stable, so ascending point order inside a subdomain), so the trained
+    pairs are a prefix.  Chunks (runs of one subdomain's pairs, one partial
+    gradient row each) end at every subdomain boundary, every `chunk` pairs
+    inside a subdomain when `chunk` is set, and at the bin edges.  fbins /
+    bbins: the forward / backward launch as that many workgroups, each over
+    an equal share of the pairs (the backward over the trained prefix only),
+    a contiguous run of chunks per workgroup; None = one workgroup per chunk.
+    Segments are the unique (point, pou) rows; pair_slot gives each
+    tile-order pair its slot in segment order (a segment's pairs are
+    contiguous there, in the reference's pair order)."""
     m_take, n_take, p_take, np_take, npou = takes
     s = len(m_take)
     n_all = len(all_ims)
     order = np.argsort(m_take, kind="stable")
     counts = np.bincount(m_take, minlength=n_all)
-    nch = (counts + chunk - 1) // chunk
-    chunk_pos = np.repeat(np.arange(n_all), nch)                    # all_ims position of each chunk
-    first = np.concatenate([[0], np.cumsum(nch)])[:-1]              # first chunk of each position
-    model_start = np.concatenate([[0], np.cumsum(counts)])
-    within = np.arange(len(chunk_pos)) - first[chunk_pos]
-    chunk_start = np.concatenate([model_start[chunk_pos] + within * chunk, [s]]).astype(np.int32)
+    model_start = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    s_train = int(model_start[n_active])
+    cuts = [model_start]
+    if chunk is not None:
+        chunk = int(chunk)
+        nch = (counts + chunk - 1) // chunk
+        pos = np.repeat(np.arange(n_all), nch)
+        within = np.arange(len(pos)) - np.concatenate([[0], np.cumsum(nch)])[:-1][pos]
+        cuts.append(model_start[pos] + within * chunk)
+    fedges = bedges = None
+    if fbins is not None:
+        fedges = np.round(np.linspace(0, s, int(fbins) + 1)).astype(np.int64)
+        cuts.append(fedges)
+    if bbins is not None:
+        bedges = np.round(np.linspace(0, s_train, int(bbins) + 1)).astype(np.int64)
+        cuts.append(bedges)
+    bounds = np.unique(np.concatenate(cuts))
+    bounds = bounds[(bounds >= 0) & (bounds <= s)]
+    if len(bounds) == 0 or bounds[-1] != s:
+        bounds = np.append(bounds, s)
+    chunk_start = bounds.astype(np.int32)                            # [n_chunks + 1], first 0, last s
+    n_chunks = len(chunk_start) - 1
+    # the subdomain position holding each chunk (empty positions repeat a start; take the last)
+    chunk_pos = np.searchsorted(model_start, chunk_start[:-1], side="right") - 1
     chunk_model = np.asarray(all_ims, dtype=np.int64)[chunk_pos].astype(np.int32)
     chunk_train = (chunk_pos < n_active).astype(np.uint8)
-    row_chunk_start = np.concatenate([first[:n_active], [first[n_active] if n_active < n_all else len(chunk_pos)]])
+    row_chunk_start = np.searchsorted(chunk_pos, np.arange(n_active + 1), side="left").astype(np.int32)
     rows = np.asarray(all_ims[:n_active], dtype=np.int32)
+    one_per_chunk = np.arange(n_chunks + 1, dtype=np.int32)
+    fbin_chunk_start = one_per_chunk if fedges is None else np.searchsorted(chunk_start, fedges).astype(np.int32)
+    bbin_chunk_start = one_per_chunk if bedges is None else np.searchsorted(chunk_start, bedges).astype(np.int32)
 
     tile_pos = np.empty(s, dtype=np.int64)
     tile_pos[order] = np.arange(s)
@@ -187,8 +213,11 @@ def build_worklist(takes, all_ims, n_active, n_points, device, chunk=FBPX_CHUNK)
         "point_seg_start": dev(point_seg_start),
         "seg_pair_start": dev(seg_pair_start),
         "pair_slot": dev(pair_slot),
+        "fbin_chunk_start": dev(fbin_chunk_start),
+        "bbin_chunk_start": dev(bbin_chunk_start),
     }
-    sizes = {"n_pairs": s, "n_chunks": len(chunk_pos), "n_points": n_points, "n_segs": n_segs, "npou": npou}
+    sizes = {"n_pairs": s, "n_chunks": n_chunks, "n_points": n_points, "n_segs": n_segs, "npou": npou,
+             "n_fbins": len(fbin_chunk_start) - 1, "n_bbins": len(bbin_chunk_start) - 1}
     wl = Worklist(tensors, sizes)
     return WorkPlan(wl, dev(chunk_train, torch.uint8), dev(rows), dev(row_chunk_start), n_active,
                     np.asarray(all_ims))

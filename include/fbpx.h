@@ -53,10 +53,16 @@ typedef struct {
  * Work list for one set of takes.  Pairs are (point, subdomain) memberships.
  * "Tile order": pairs grouped by subdomain (ascending chunk index), within a
  * subdomain in ascending point order.  Each chunk is a contiguous run of
- * pairs of ONE subdomain, one workgroup each (any length >= 1; FBPX_CHUNK is
- * the default, the engine's autotuner may pick another per shape).  "Segment": a unique
- * (point, pou) row of the reference's jnp.unique (trainers.py:380-385),
- * ordered by (point, pou).
+ * pairs of ONE subdomain (any length >= 1) and produces one partial gradient
+ * row.  "Bins": the forward and the backward each launch one workgroup per
+ * bin; a bin is a contiguous range of chunks (fbin_chunk_start /
+ * bbin_chunk_start), processed in order by that workgroup.  The engine cuts
+ * the pairs into as many equal bins as the GPU holds workgroups (one
+ * persistent launch per kernel, no tail), with chunk boundaries at the
+ * subdomain boundaries and both kernels' bin edges; one bin per chunk of at
+ * most FBPX_CHUNK pairs is the plain one-workgroup-per-chunk launch.
+ * "Segment": a unique (point, pou) row of the reference's jnp.unique
+ * (trainers.py:380-385), ordered by (point, pou).
  */
 #define FBPX_CHUNK 512
 typedef struct {
@@ -65,6 +71,8 @@ typedef struct {
   int32_t n_points;
   int32_t n_segs;
   int32_t npou;
+  int32_t n_fbins;                /* forward workgroups */
+  int32_t n_bbins;                /* backward workgroups */
   const int32_t* pair_point;      /* [n_pairs] tile order -> point index */
   const int32_t* pair_seg;        /* [n_pairs] tile order -> segment index */
   const int32_t* chunk_model;     /* [n_chunks] subdomain row of the chunk */
@@ -75,6 +83,9 @@ typedef struct {
                                      pairs of segment s occupy slots
                                      seg_pair_start[s] .. seg_pair_start[s+1]-1, in the
                                      reference's pair order */
+  const int32_t* fbin_chunk_start; /* [n_fbins+1] chunks of forward workgroup b:
+                                      fbin_chunk_start[b] .. fbin_chunk_start[b+1]-1 */
+  const int32_t* bbin_chunk_start; /* [n_bbins+1] chunks of backward workgroup b */
 } fbpx_worklist;
 
 /* ---------------------------------------------------------------- takes */
@@ -112,7 +123,8 @@ int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom,
  * i.e. the vmapped FBPINN_model_inner (trainers.py:113-118,158) with the
  * nested-jvp tangents of _get_ujs (trainers.py:213-247) carried in registers.
  * theta: [m][P]; pairbuf: [n_pairs][J], row = the pair's slot (segment
- * order), so that fbpx_pou_reduce reads each segment's rows contiguously. */
+ * order), so that fbpx_pou_reduce reads each segment's rows contiguously.
+ * One workgroup per forward bin; the bins must cover every chunk. */
 int fbpx_forward(void* stream, const fbpx_net* net, const float* theta,
                  const float* dom, const float* x, const fbpx_worklist* wl,
                  float* pairbuf);
@@ -132,12 +144,20 @@ int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl
 
 /* Reverse pass through the jets to the subdomain weights (the part of
  * value_and_grad, trainers.py:292, below the segment sums).  Recomputes
- * activations.  Chunks with chunk_train[c] == 0 (fixed subdomains,
- * trainers.py:252-256 -- no gradient) are skipped and their partial row is
- * left untouched.  partial: [n_chunks][P] per-chunk gradient rows. */
+ * activations.  One workgroup per backward bin; chunks outside the bins or
+ * with chunk_train[c] == 0 (fixed subdomains, trainers.py:252-256 -- no
+ * gradient) are skipped and their partial row is left untouched.
+ * partial: [n_chunks][P] per-chunk gradient rows. */
 int fbpx_backward(void* stream, const fbpx_net* net, const float* theta,
                   const float* dom, const float* x, const fbpx_worklist* wl,
                   const uint8_t* chunk_train, const float* gseg, float* partial);
+
+/* Resident 256-thread workgroups per compute unit of fbpx_forward's and
+ * fbpx_backward's kernel for this network and jet schema (register and LDS
+ * limits), for sizing the persistent launches (one workgroup per bin).  No
+ * counterpart in the reference: its XLA compilation picks launch shapes
+ * itself. */
+int fbpx_occupancy(const fbpx_net* net, int32_t* fwd_per_cu, int32_t* bwd_per_cu);
 
 /* Per-row gradient from the per-chunk partial rows: for k < n_rows,
  *   grad[rows[k]][e] = sum over c in row_chunk_start[k] .. row_chunk_start[k+1]-1

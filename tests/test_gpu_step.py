@@ -197,27 +197,36 @@ def test_graph_replay_equals_eager(name):
         assert int(ea.count.item()) == int(eb.count.item()) == k + 1
 
 
-def test_chunk_size_changes_only_summation_order(monkeypatch):
-    """Every autotuner candidate chunk size gives the same loss (the pair jets
-    do not depend on the chunking) and gradients equal to float32 summation
-    accuracy; the autotuner returns one of the candidates."""
+def test_launch_shape_changes_only_summation_order(monkeypatch):
+    """Every launch shape -- one workgroup per chunk of any size, persistent
+    bins per kernel, mixed -- gives the same loss (the pair jets do not depend
+    on the launch) and gradients equal to float32 summation accuracy, also with
+    fixed subdomains (the backward's bins cover the trained pairs only); the
+    autotuner returns candidates."""
     from fbpinns_amd import autotune
     monkeypatch.setattr(autotune, "MIN_PAIRS", 0)
     autotune._cache.clear()
     eng, *_ = _build("wave_small")
-    print(f"[chunk] autotuned chunk {eng.chunk}, pairs {eng.plan.wl.n_pairs}", flush=True)
-    assert eng.chunk in autotune.CANDIDATES
-    ones = np.ones(eng.m, dtype=int)
-    eng.set_active(ones, chunk=512)
-    l0, g0, _ = eng.loss_and_grad()
-    for c in (1, 64, 256, 1024, 2048):
-        eng.set_active(ones, chunk=c)
-        assert eng.plan.wl.n_chunks >= eng.plan.wl.n_pairs // max(c, 1)
-        l1, g1, _ = eng.loss_and_grad()
-        e = ((g1 - g0).abs().max() / g0.abs().max()).item()
-        print(f"  chunk {c}: chunks {eng.plan.wl.n_chunks} loss {l1.item():.8e} grad rel {e:.2e}", flush=True)
-        assert l1.item() == l0.item()
-        assert e < 1e-5
+    print(f"[launch] autotuned {eng.launch}, pairs {eng.plan.wl.n_pairs}, "
+          f"occupancy {eng.net.occupancy()}", flush=True)
+    assert eng.launch[0] in autotune.CANDIDATES and eng.launch[1] in autotune.CANDIDATES
+    shapes = [(("chunks", c), ("chunks", c)) for c in (1, 64, 256, 1024, 2048)]
+    shapes += [(("bins", 1), ("bins", 1)), (("bins", 2), ("bins", 3)), (("chunks", 512), ("bins", 1)),
+               (("bins", 1), ("chunks", 64))]
+    for active in (np.ones(eng.m, dtype=int), np.where(np.arange(eng.m) % 3 == 0, 2, 1)):
+        eng.set_active(active, chunk=512)
+        l0, g0, _ = eng.loss_and_grad()
+        for launch in shapes:
+            eng.set_active(active, launch=launch)
+            wl = eng.plan.wl
+            l1, g1, _ = eng.loss_and_grad()
+            e = ((g1 - g0).abs().max() / g0.abs().max()).item()
+            print(f"  {launch}: chunks {wl.n_chunks} workgroups {wl.n_fbins}/{wl.n_bbins} "
+                  f"loss {l1.item():.8e} grad rel {e:.2e}", flush=True)
+            assert l1.item() == l0.item()
+            assert e < 1e-5
+            if launch[1][0] == "chunks" and launch[0][0] == "chunks":
+                assert wl.n_chunks >= wl.n_pairs // launch[0][1]
 
 
 def test_scheduler_masks_fixed_rows():

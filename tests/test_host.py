@@ -105,6 +105,56 @@ def test_build_worklist_invariants():
     assert pss[-1] == len(np_take) and np.all(np.diff(pss) >= 1)
 
 
+def test_build_worklist_bins():
+    """Persistent launch bins: the forward's bins split all pairs, the
+    backward's the trained prefix, each into equal shares (+-1 pair) of whole
+    chunks; chunk boundaries fall on subdomain boundaries and bin edges (and
+    every `chunk` pairs when a cap is given)."""
+    r = np.random.default_rng(6)
+    _, dec = setups.decomposition_params((9, 4), (0.0, 0.0), (1.0, 1.0), 2.5, multilevel=False)
+    lo, hi = dec["subdomain"]["params"][0].numpy(), dec["subdomain"]["params"][1].numpy()
+    pou = dec["subdomain"]["pou"].numpy().reshape(-1)
+    m = lo.shape[0]
+    x = r.uniform(0, 1, (2500, 2)).astype(np.float32)
+    active = np.ones(m, int)
+    active[::3] = 2
+    takes, all_ims, act, a_ims, f_ims = ot.get_inputs(x, active, lo, hi, pou)
+    s = len(takes[0])
+    counts = np.bincount(takes[0], minlength=len(all_ims))
+    s_train = int(counts[:len(a_ims)].sum())
+    model_start = np.concatenate([[0], np.cumsum(counts)])
+    for cap, nf, nb in ((None, 7, 5), (None, 1, 1), (64, 13, 3), (None, 300, 200)):
+        plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=cap, fbins=nf, bbins=nb)
+        t = {k: v.numpy() for k, v in plan.wl.t.items()}
+        cs, fb, bb = t["chunk_start"], t["fbin_chunk_start"], t["bbin_chunk_start"]
+        n_chunks = plan.wl.n_chunks
+        assert plan.wl.n_fbins == nf and plan.wl.n_bbins == nb
+        assert cs[0] == 0 and cs[-1] == s and np.all(np.diff(cs) > 0) and len(cs) == n_chunks + 1
+        if cap is not None:
+            assert np.all(np.diff(cs) <= cap)
+        assert set(model_start.tolist()) <= set(cs.tolist())
+        glob = all_ims[takes[0]]
+        order = np.argsort(takes[0], kind="stable")
+        assert np.array_equal(np.repeat(t["chunk_model"], np.diff(cs)), glob[order])
+        # forward bins: all chunks, equal pair shares
+        assert fb[0] == 0 and fb[-1] == n_chunks and np.all(np.diff(fb) >= 0)
+        assert np.all(np.abs(np.diff(cs[fb]) - s / nf) <= 1)
+        # backward bins: exactly the trained chunks, equal shares of the trained pairs
+        n_train_chunks = int(plan.chunk_train.numpy().sum())
+        assert cs[n_train_chunks] == s_train
+        assert bb[0] == 0 and bb[-1] == n_train_chunks and np.all(np.diff(bb) >= 0)
+        assert np.all(np.abs(np.diff(cs[bb]) - s_train / nb) <= 1)
+        rcs = plan.row_chunk_start.numpy()
+        for k, row in enumerate(a_ims):
+            assert set(t["chunk_model"][rcs[k]:rcs[k + 1]]) <= {row}
+            assert cs[rcs[k + 1]] - cs[rcs[k]] == counts[k]
+    # no bins: one workgroup per chunk for both kernels
+    plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
+    n_chunks = plan.wl.n_chunks
+    assert np.array_equal(plan.wl.t["fbin_chunk_start"].numpy(), np.arange(n_chunks + 1))
+    assert np.array_equal(plan.wl.t["bbin_chunk_start"].numpy(), np.arange(n_chunks + 1))
+
+
 # ------------------------------------------------------------------ epilogue
 
 def test_get_jmaps_matches_oracle():
