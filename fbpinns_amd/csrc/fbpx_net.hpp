@@ -789,8 +789,15 @@ __device__ inline void w2t_pass(const LaneLds& v, const f4* G, f4* GA) {
 
 // Workgroup-shared dW2 accumulation.  The 4 waves of a workgroup process
 // sub-tiles in lockstep (64 pairs per tile) and write two transposes to LDS:
-//   stb[((rb*4 + w)*16 + p)*16 + s] = layer-1 h of unit 16rb+s, pair p of wave w
-//   stg[((ob*4 + w)*16 + p)*16 + s] = G_c of unit 16ob+s, pair p of wave w
+//   stb[rb*BS + w*WS + (p >> 2)*PG + (p & 3)*16 + s] = layer-1 h of unit 16rb+s,
+//                                                      pair p of wave w
+//   stg[ob*BS + ...same...]                          = G_c of unit 16ob+s
+// For h <= 32 each group of 4 pairs is padded from 64 to PG = 68 floats, so
+// the 16 lanes of a 128-bit store phase (pairs 0..15, 16 bytes each) start
+// in 16 distinct 4-bank groups instead of sharing 4 of them; a read (one pair
+// group, 64 lanes) stays 64 consecutive floats.  At h = 64 the unpadded
+// offsets (multiples of 64 floats) let the reads pair up as ds_read2st64,
+// and the registers that saves are worth more than the store conflicts.
 // dW2 blocks (16 x 16, rows o, columns u) are owned by waves: H = 64: wave w
 // owns column block w (all 4 row blocks); H = 32: block (w >> 1, w & 1);
 // H = 16: every wave holds the single block over its own pairs.  A wave
@@ -801,7 +808,10 @@ struct DW2Own {
   static constexpr int NRB = H / 16;
   static constexpr int NOB = NRB == 4 ? 4 : 1;     // row blocks held by a wave
   static constexpr int NSRC = NRB == 1 ? 1 : 4;    // waves whose pairs it contracts over
-  static constexpr int TSZ = NRB * 4 * 256;         // floats per transpose buffer
+  static constexpr int PG = H == 64 ? 64 : 68;      // floats per group of 4 pairs
+  static constexpr int WS = 4 * PG;                 // per wave (16 pairs)
+  static constexpr int BS = 4 * WS;                 // per 16-unit block (4 waves)
+  static constexpr int TSZ = NRB * BS;              // floats per transpose buffer
   __device__ static int col_block(int wave) { return NRB == 4 ? wave : (NRB == 2 ? (wave & 1) : 0); }
   __device__ static int row_block(int wave, int j) { return NRB == 4 ? j : (NRB == 2 ? (wave >> 1) : 0); }
 };
@@ -809,9 +819,9 @@ struct DW2Own {
 // dW2[j] += G_c a1_c^T over the tile.  a1_c of unit u = 16 ibw + pl is
 // rebuilt from h: KIND 0: h, 1: (1 - h^2) sc, 2: h (1 - h^2) sc with sc
 // supplied by the caller (zt_i and -2 zt_i^2 for unit u).
-// stgl = stg + 16 g + pl + (NRB == 2 ? row block (wave >> 1) * 1024 : 0) and
-// stbl = stb + (ibw * 64 + g) * 16 + pl are this lane's read bases (wave w,
-// pair 4ks + g at offset (16 w + 4 ks) * 16, row block ob at ob * 1024).
+// stgl = stg + 16 g + pl + (NRB == 2 ? row block (wave >> 1) * BS : 0) and
+// stbl = stb + ibw * BS + 16 g + pl are this lane's read bases (wave w,
+// pair 4ks + g at offset w * WS + ks * PG, row block ob at ob * BS).
 template <int H, int KIND>
 __device__ inline void dw2_shared(const float* stgl, const float* stbl, float sc, f4* dW2, int wave) {
   using O = DW2Own<H>;
@@ -820,13 +830,13 @@ __device__ inline void dw2_shared(const float* stgl, const float* stbl, float sc
     const int w = O::NSRC == 1 ? wave : s;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      const float h = stbl[(16 * w + 4 * ks) * 16];
+      const float h = stbl[w * O::WS + ks * O::PG];
       float b = h;
       if (KIND == 1) b = fmaf(-h, h, 1.f) * sc;
       if (KIND == 2) b = h * fmaf(-h, h, 1.f) * sc;
 #pragma unroll
       for (int j = 0; j < O::NOB; ++j) {
-        const float a = stgl[(O::NOB == 1 ? 0 : j * 1024) + (16 * w + 4 * ks) * 16];
+        const float a = stgl[(O::NOB == 1 ? 0 : j * O::BS) + w * O::WS + ks * O::PG];
         dW2[j] = mfma16(a, b, dW2[j]);
       }
     }
@@ -834,11 +844,11 @@ __device__ inline void dw2_shared(const float* stgl, const float* stbl, float sc
 }
 
 // Write this wave's D-layout tile (units x 16 pairs) into a transpose buffer;
-// tl = t + (16 wave + pl) * 16 + 4 g.
+// tl = t + wave * WS + (pl >> 2) * PG + (pl & 3) * 16 + 4 g.
 template <int H>
 __device__ inline void store_tile(float* tl, const f4* v) {
 #pragma unroll
-  for (int ob = 0; ob < H / 16; ++ob) *reinterpret_cast<f4*>(tl + ob * 1024) = v[ob];
+  for (int ob = 0; ob < H / 16; ++ob) *reinterpret_cast<f4*>(tl + ob * DW2Own<H>::BS) = v[ob];
 }
 
 // [xd, h, h, 1]: hidden-to-hidden layer on MFMA, streamed one axis at a time.
@@ -889,10 +899,11 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   const int ucol = 16 * O::col_block(wave) + pl;   // this lane's dW2 column unit
   const float wl_red = ss[SM::WL + 16 * (pl >> 2) + 4 * g + (pl & 3)];   // WL of the unit a reduction leaves here
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
-  float* const stbw = stb + (16 * wave + pl) * 16 + 4 * g;          // store bases
-  float* const stgw = stg + (16 * wave + pl) * 16 + 4 * g;
-  const float* const stbl = stb + (O::col_block(wave) * 64 + g) * 16 + pl;   // dW2 read bases
-  const float* const stgl = stg + (H == 32 ? (wave >> 1) * 1024 : 0) + 16 * g + pl;
+  const int st_lane = wave * O::WS + (pl >> 2) * O::PG + (pl & 3) * 16 + 4 * g;
+  float* const stbw = stb + st_lane;                                 // store bases
+  float* const stgw = stg + st_lane;
+  const float* const stbl = stb + O::col_block(wave) * O::BS + 16 * g + pl;   // dW2 read bases
+  const float* const stgl = stg + (H == 32 ? (wave >> 1) * O::BS : 0) + 16 * g + pl;
   float* mysmall = ssmall + wave * NS * H;
   f4 dW2[O::NOB];
 #pragma unroll
