@@ -94,7 +94,7 @@ class Constants(ConstantsBase):
         self.save_figures = False
         self.clear_output = False
         self.graph = True          # capture the whole train step in a hipGraph
-        self.autotune = True       # time candidate chunk sizes per work-list shape
+        self.autotune = True       # time candidate launch shapes per work-list shape
         self.device = "cuda:0"
         self.hostname = socket.gethostname().lower()
         for key in kwargs.keys():

@@ -1,7 +1,8 @@
 // Forward and backward jet kernels over (point, subdomain) pairs.
 //
-// Work decomposition: one workgroup (4 waves) per chunk (a run of pairs of a
-// single subdomain, FBPX_CHUNK by default); each wave processes 16 pairs at a time ("sub-tile").
+// Work decomposition: one workgroup (4 waves) per bin, a bin being a run of
+// chunks (a chunk = a run of pairs of a single subdomain) processed in order;
+// each wave processes 16 pairs at a time ("sub-tile").
 // Lane l handles pair (l & 15) of the sub-tile and hidden units
 //   u = 16*rb + 4*(l >> 4) + r,   rb < H/16, r < 4,
 // which is exactly the row set that lane holds in the D layout of
