@@ -14,6 +14,8 @@ FBPX_DOM_STRIDE = 12
 FBPX_MAX_XD = 3
 FBPX_CHUNK = 512
 FBPX_MAX_REQ = 16
+FBPX_MAX_TERMS = 8
+FBPX_SUM_BLOCKS = 256
 
 _c_i32 = ctypes.c_int32
 _c_p = ctypes.c_void_p
@@ -62,6 +64,12 @@ SIGNATURES = {
                                   ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p)]),
     "fbpx_affine_jets_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_i32, _c_i32, _c_p, _c_i32,
                                            ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p), _c_p]),
+    "fbpx_residual_loss": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
+                                    ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, ctypes.c_float, _c_p,
+                                    _c_p, _c_p]),
+    "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
+                                             ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, _c_p,
+                                             ctypes.c_float, _c_p, ctypes.POINTER(_c_p), _c_p, _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_last_error": (ctypes.c_char_p, []),

@@ -401,6 +401,91 @@ __global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict
   if (threadIdx.x == 0) out[0] = t / (float)n;
 }
 
+// ------------------------------------------------------------------ fused residual loss
+
+// Column and term tables of one residual; d*: per column f, the terms in
+// which it appears, as (term, other factor column or -1) in
+// dterm/dother[dstart[f] .. dstart[f+1]-1] (a term col_f * col_f appears twice).
+struct ResidualArgs {
+  int n_cols, n_terms;
+  const float* col[FBPX_MAX_REQ];
+  float* gcol[FBPX_MAX_REQ];
+  int fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
+  int dstart[FBPX_MAX_REQ + 1];
+  int dterm[2 * FBPX_MAX_TERMS], dother[2 * FBPX_MAX_TERMS];
+};
+
+__device__ inline float residual_term(const ResidualArgs& a, int t, int64_t k) {
+  const float x = a.fa[t] >= 0 ? a.col[a.fa[t]][k] : 1.f;
+  return a.fb[t] >= 0 ? x * a.col[a.fb[t]][k] : x;
+}
+
+// r[k] = sum_t coef[t] term_t(k); scratch[block] = sum of r^2 over the
+// block's grid-stride points.
+__global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
+                                                       float* __restrict__ r, float* __restrict__ scratch) {
+  __shared__ float sm[256];
+  float s = 0.f;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    float v = 0.f;
+#pragma unroll
+    for (int t = 0; t < FBPX_MAX_TERMS; ++t)
+      if (t < a.n_terms) v = fmaf(coef[t], residual_term(a, t, k), v);
+    r[k] = v;
+    s = fmaf(v, v, s);
+  }
+  const float tot = block_tree_sum(s, sm);
+  if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
+}
+
+// gcol[f][k] = s r[k] d r / d col_f, s = g_loss[0] * w2n; scratch[block *
+// n_terms + t] = sum over the block's points of s r term_t.
+__global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, ResidualArgs a,
+                                                                const float* __restrict__ coef,
+                                                                const float* __restrict__ r, float w2n,
+                                                                const float* __restrict__ g_loss,
+                                                                float* __restrict__ scratch) {
+  __shared__ float sm[256];
+  const float sc = g_loss[0] * w2n;
+  float acc[FBPX_MAX_TERMS];
+#pragma unroll
+  for (int t = 0; t < FBPX_MAX_TERMS; ++t) acc[t] = 0.f;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const float sr = sc * r[k];
+    for (int f = 0; f < a.n_cols; ++f) {
+      if (a.gcol[f] == nullptr) continue;
+      float g = 0.f;
+      for (int e = a.dstart[f]; e < a.dstart[f + 1]; ++e) {
+        const float other = a.dother[e] >= 0 ? a.col[a.dother[e]][k] : 1.f;
+        g = fmaf(coef[a.dterm[e]], other, g);
+      }
+      a.gcol[f][k] = sr * g;
+    }
+#pragma unroll
+    for (int t = 0; t < FBPX_MAX_TERMS; ++t)
+      if (t < a.n_terms) acc[t] = fmaf(sr, residual_term(a, t, k), acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
+    if (t >= a.n_terms) break;
+    const float tot = block_tree_sum(acc[t], sm);
+    if (threadIdx.x == 0) scratch[blockIdx.x * a.n_terms + t] = tot;
+    __syncthreads();
+  }
+}
+
+// out[j] = scale * sum over b < nb of scratch[b * ncol + j], j < ncol (one block).
+__global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restrict__ scratch, int nb, int ncol,
+                                                          float scale, float* __restrict__ out) {
+  __shared__ float sm[256];
+  for (int j = 0; j < ncol; ++j) {
+    const float v = (int)threadIdx.x < nb ? scratch[threadIdx.x * ncol + j] : 0.f;
+    const float t = block_tree_sum(v, sm);
+    if (threadIdx.x == 0) out[j] = scale * t;
+    __syncthreads();
+  }
+}
+
 __global__ void count_increment_kernel(int32_t* count) {
   if (threadIdx.x == 0) count[0] = count[0] + 1;
 }
@@ -645,6 +730,89 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
   hipLaunchKernelGGL(affine_jets_backward_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
                      ld, J, B, a, gq);
   return hip_check(hipGetLastError(), "affine_jets_backward");
+}
+
+static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms, const int32_t* fa,
+                         const int32_t* fb, float* const* gcol, ResidualArgs* a) {
+  if (n <= 0 || n_cols < 1 || n_cols > FBPX_MAX_REQ || n_terms < 1 || n_terms > FBPX_MAX_TERMS || !col || !fa ||
+      !fb) {
+    set_error("residual loss: bad sizes n=%lld n_cols=%d n_terms=%d (at most %d columns, %d terms)", (long long)n,
+              n_cols, n_terms, FBPX_MAX_REQ, FBPX_MAX_TERMS);
+    return FBPX_EINVAL;
+  }
+  *a = ResidualArgs{};
+  a->n_cols = n_cols;
+  a->n_terms = n_terms;
+  for (int f = 0; f < n_cols; ++f) {
+    if (!col[f]) {
+      set_error("residual loss: null column %d", f);
+      return FBPX_EINVAL;
+    }
+    a->col[f] = col[f];
+    a->gcol[f] = gcol ? gcol[f] : nullptr;
+  }
+  for (int t = 0; t < n_terms; ++t) {
+    if (fa[t] >= n_cols || fb[t] >= n_cols) {
+      set_error("residual loss: term %d factors (%d, %d) outside %d columns", t, fa[t], fb[t], n_cols);
+      return FBPX_EINVAL;
+    }
+    a->fa[t] = fa[t] < 0 ? -1 : fa[t];
+    a->fb[t] = fb[t] < 0 ? -1 : fb[t];
+  }
+  int e = 0;
+  for (int f = 0; f < n_cols; ++f) {
+    a->dstart[f] = e;
+    for (int t = 0; t < n_terms; ++t) {
+      if (a->fa[t] == f) {
+        a->dterm[e] = t;
+        a->dother[e++] = a->fb[t];
+      }
+      if (a->fb[t] == f) {
+        a->dterm[e] = t;
+        a->dother[e++] = a->fa[t];
+      }
+    }
+  }
+  a->dstart[n_cols] = e;
+  return FBPX_OK;
+}
+
+int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
+                       const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
+                       float* scratch, float* loss) {
+  ResidualArgs a;
+  int rc = residual_args(n, n_cols, col, n_terms, fa, fb, nullptr, &a);
+  if (rc) return rc;
+  if (!coef || !r || !scratch || !loss) {
+    set_error("residual loss: null coef / r / scratch / loss");
+    return FBPX_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t need = (n + 255) / 256;
+  const int nb = (int)(need < kSumBlocks ? need : kSumBlocks);
+  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch);
+  hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, weight / (float)n, loss);
+  return hip_check(hipGetLastError(), "residual_loss");
+}
+
+int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                                int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
+                                const float* r, float weight, const float* g_loss, float* const* gcol,
+                                float* scratch, float* gcoef) {
+  ResidualArgs a;
+  int rc = residual_args(n, n_cols, col, n_terms, fa, fb, gcol, &a);
+  if (rc) return rc;
+  if (!coef || !r || !g_loss || !scratch) {
+    set_error("residual loss backward: null coef / r / g_loss / scratch");
+    return FBPX_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t need = (n + 255) / 256;
+  const int nb = (int)(need < kSumBlocks ? need : kSumBlocks);
+  hipLaunchKernelGGL(residual_backward_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, 2.f * weight / (float)n,
+                     g_loss, scratch);
+  if (gcoef) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
+  return hip_check(hipGetLastError(), "residual_loss_backward");
 }
 
 int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out) {

@@ -252,7 +252,8 @@ class FusedEngine:
                     f"fwd {autotune.name(self.launch[0])} ({plan.wl.n_fbins} workgroups), "
                     f"bwd {autotune.name(self.launch[1])} ({plan.wl.n_bbins} workgroups), "
                     f"{n_active} trained / {len(all_ims)} used subdomains, jets J={self.net.J}, "
-                    f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}, d={dstat:.2f}")
+                    f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}"
+                    f"{' + fused residual loss' if self.epilogue.fused is not None else ''}, d={dstat:.2f}")
         self._graph = None
         if self.graph_enabled:
             self._capture()

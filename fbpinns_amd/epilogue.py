@@ -18,13 +18,19 @@ kernels, and only the cheap constraining function is differentiated:
   the same derivatives for every request of order <= 2 with repeated indices.
 
 Then loss_fn(all_params, constraints_) is evaluated exactly as the reference
-calls it (trainers.py:258-267).
+calls it (trainers.py:258-267), or, when the problem describes its loss as
+weighted means of squared pointwise residuals (fbpinns_amd.residuals) and the
+description agrees with loss_fn at set-up, one fused kernel pair per residual
+(fbpx_residual_loss, ops.ResidualLoss) evaluates it on the GPU.
 """
 
 import torch
 from torch.func import jvp
 
-from fbpinns_amd.ops import AffineJets, FbpxError
+from fbpinns_amd._lib import FBPX_MAX_REQ
+from fbpinns_amd.ops import AffineJets, FbpxError, ResidualLoss
+from fbpinns_amd.residuals import F, J, V, check_spec, spec_loss
+from fbpinns_amd.util.logger import logger
 
 
 # ------------------------------------------------------------------ derivative tree
@@ -119,13 +125,14 @@ class Epilogue:
     cvalues    per constraint, the list [x_c, *values_c] passed to loss_fn
     """
 
-    def __init__(self, problem, all_params, x, cranges, cvalues, required_ujss, net):
+    def __init__(self, problem, all_params, x, cranges, cvalues, required_ujss, net, fused_loss=True):
         self.problem, self.net = problem, net
         self.x, self.cranges, self.cvalues = x, cranges, cvalues
         self.required_ujss = [tuple((iu, tuple(ixs)) for iu, ixs in r) for r in required_ujss]
         self.jmapss = [get_jmaps(r) for r in self.required_ujss]
         self.col = _col_index(net)
         self.affine = self._affine_precompute(all_params)
+        self.fused = self._fused_precompute(all_params) if fused_loss else None
 
     # -------------------------------------------------------------- set-up
 
@@ -184,6 +191,103 @@ class Epilogue:
         B_cols[order] = Bj
         return A_cols, B_cols
 
+    def _fused_precompute(self, all_params):
+        """Residual description of loss_fn (problem.residuals), checked against
+        loss_fn in float64, with the per-point data columns and static
+        coefficients it needs each step; None selects loss_fn itself."""
+        fn = getattr(self.problem, "residuals", None)
+        if fn is None or not self.x.is_cuda:
+            return None
+        n_req = [len(r) for r in self.required_ujss]
+        n_values = [len(c) - 1 for c in self.cvalues]
+        try:
+            spec = fn(all_params)
+            if spec is None:
+                return None
+            check_spec(spec, len(self.cranges), n_req, n_values)
+            if any(self.cranges[res.constraint][1] == 0 for res in spec):
+                return None
+            if not self._fused_agrees(spec, all_params, n_values):
+                logger.info("fused residual loss: description disagrees with loss_fn, using loss_fn")
+                return None
+        except Exception as e:  # noqa: BLE001 - any failure selects loss_fn
+            logger.info(f"fused residual loss not used: {e}")
+            return None
+        plans = []
+        for res in spec:
+            ic = res.constraint
+            x_c = self.cvalues[ic][0]
+            n_c = self.cranges[ic][1]
+            keys, data, fa, fb = [], {}, [], []
+
+            def slot(f):
+                if f is None:
+                    return -1
+                key = ("J", f.i) if isinstance(f, J) else ("V", f.j) if isinstance(f, V) else ("F", id(f))
+                if key not in keys:
+                    keys.append(key)
+                    if isinstance(f, V):
+                        data[key] = self.cvalues[ic][1 + f.j].to(torch.float32).reshape(n_c).contiguous()
+                    elif isinstance(f, F):
+                        with torch.no_grad():
+                            v = f.fn(all_params, x_c.double())
+                        data[key] = torch.as_tensor(v).to(torch.float64).expand(n_c, 1).reshape(n_c) \
+                            .to(torch.float32).contiguous()
+                return keys.index(key)
+            for _, a, b in res.terms:
+                fa.append(slot(a))
+                fb.append(slot(b))
+            if len(keys) > FBPX_MAX_REQ:
+                return None
+            static = torch.tensor([0.0 if callable(c) else float(c) for c, _, _ in res.terms], dtype=torch.float32,
+                                  device=self.x.device)
+            dyn = [(t, c) for t, (c, _, _) in enumerate(res.terms) if callable(c)]
+            plans.append((ic, float(res.weight), keys, data, fa, fb, static, dyn))
+        return plans
+
+    def _fused_agrees(self, spec, all_params, n_values):
+        """loss_fn and the residual description at random derivatives, in
+        float64: values and gradients (derivatives, trainable parameters)."""
+        probe = _with_grad_trainable(all_params)
+        leaves = _trainable_leaves(probe)
+        g = torch.Generator(device="cpu").manual_seed(4321)
+        cons, jets = [], []
+        for ic, (off, n_c) in enumerate(self.cranges):
+            vals = [v.double() if isinstance(v, torch.Tensor) and v.is_floating_point() else v
+                    for v in self.cvalues[ic]]
+            js = [torch.randn((n_c, 1), generator=g, dtype=torch.float64).to(self.x.device).requires_grad_(True)
+                  for _ in self.required_ujss[ic]]
+            cons.append(vals + js)
+            jets += js
+        a = self.problem.loss_fn(probe, cons)
+        b = spec_loss(spec, probe, cons, n_values)
+        wrt = jets + leaves
+        ga = torch.autograd.grad(a, wrt, allow_unused=True)
+        gb = torch.autograd.grad(b, wrt, allow_unused=True)
+        a, b = float(a.detach()), float(b.detach())
+        if not abs(a - b) <= 1e-9 * max(abs(a), 1e-300):
+            return False
+        for u, v, w in zip(ga, gb, wrt):
+            u = torch.zeros_like(w) if u is None else u
+            v = torch.zeros_like(w) if v is None else v
+            if (u - v).abs().max() > 1e-9 * (1.0 + u.abs().max()):
+                return False
+        return True
+
+    def _fused_loss(self, ujss, all_params):
+        loss = None
+        for ic, weight, keys, data, fa, fb, static, dyn in self.fused:
+            cols = [ujss[ic][k[1]].reshape(-1) if k[0] == "J" else data[k] for k in keys]
+            coef = static
+            if dyn:
+                parts = list(static.unbind())
+                for t, c in dyn:
+                    parts[t] = torch.as_tensor(c(all_params)).reshape(()).to(torch.float32)
+                coef = torch.stack(parts)
+            term = ResidualLoss.apply(coef, weight, fa, fb, *cols)
+            loss = term if loss is None else loss + term
+        return loss
+
     # -------------------------------------------------------------- per step
 
     def _ujs_generic(self, ic, q, all_params):
@@ -211,26 +315,44 @@ class Epilogue:
 
     def loss(self, ujet, all_params):
         """ujet: (J, n) jets of u~ at self.x.  Returns the scalar loss."""
-        constraints_ = []
+        ujss = []
         if self.affine is not None:
             # Leibniz rule on the jets in one kernel per constraint
             A, B = self.affine
             reqs = [self._req_codes(r) for r in self.required_ujss]
             outs = AffineJets.apply(ujet, A, B, self.net.as_list(), list(self.cranges), reqs)
             i = 0
-            for ic, req in enumerate(self.required_ujss):
-                constraints_.append(list(self.cvalues[ic]) + list(outs[i:i + len(req)]))
+            for req in self.required_ujss:
+                ujss.append(list(outs[i:i + len(req)]))
                 i += len(req)
         else:
             for ic, (off, n_c) in enumerate(self.cranges):
-                ujs = self._ujs_generic(ic, ujet[:, off:off + n_c], all_params)
-                constraints_.append(list(self.cvalues[ic]) + ujs)
+                ujss.append(self._ujs_generic(ic, ujet[:, off:off + n_c], all_params))
+        if self.fused is not None:
+            return self._fused_loss(ujss, all_params)
+        constraints_ = [list(self.cvalues[ic]) + ujss[ic] for ic in range(len(self.cranges))]
         return self.problem.loss_fn(all_params, constraints_)
 
     def u(self, ujet, all_params):
         """Constrained solution value u at self.x (value column only)."""
         q0 = ujet[0][:, None]
         return self.problem.constraining_fn(all_params, self.x, q0)
+
+
+def _trainable_leaves(all_params):
+    out = []
+
+    def rec(d):
+        if isinstance(d, dict):
+            for v in d.values():
+                rec(v)
+        elif isinstance(d, (list, tuple)):
+            for v in d:
+                rec(v)
+        elif isinstance(d, torch.Tensor) and d.requires_grad:
+            out.append(d)
+    rec(all_params["trainable"])
+    return out
 
 
 def _with_grad_trainable(all_params):

@@ -314,6 +314,93 @@ class AffineJets(torch.autograd.Function):
         return gq, None, None, None, None, None
 
 
+def _residual_tables(cols, fa, fb, n, name):
+    if not 1 <= len(cols) <= _lib.FBPX_MAX_REQ or not 1 <= len(fa) <= _lib.FBPX_MAX_TERMS or len(fb) != len(fa):
+        raise FbpxError(f"{name}: {len(cols)} columns (at most {_lib.FBPX_MAX_REQ}), {len(fa)}/{len(fb)} "
+                        f"term factors (at most {_lib.FBPX_MAX_TERMS})")
+    for t in cols:
+        _p(t, torch.float32, name)
+        if t.numel() != n:
+            raise FbpxError(f"{name}: column of {t.numel()} values for {n} points")
+    ptrs = (ctypes.c_void_p * len(cols))(*[t.data_ptr() for t in cols])
+    a = (ctypes.c_int32 * len(fa))(*[int(v) for v in fa])
+    b = (ctypes.c_int32 * len(fb))(*[int(v) for v in fb])
+    return ptrs, a, b
+
+
+@torch.library.custom_op("fbpx::residual_loss", mutates_args=("r", "scratch", "loss"))
+def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor, weight: float,
+                  r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor) -> None:
+    n = r.numel()
+    ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss")
+    if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_SUM_BLOCKS or loss.numel() != 1:
+        raise FbpxError(f"residual_loss: coef {coef.numel()} for {len(fa)} terms, scratch {scratch.numel()}, "
+                        f"loss {loss.numel()}")
+    check(lib().fbpx_residual_loss(_stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
+                                   float(weight), _p(r, torch.float32, "r"), _p(scratch, torch.float32, "scratch"),
+                                   _p(loss, torch.float32, "loss")), "fbpx_residual_loss")
+
+
+@torch.library.custom_op("fbpx::residual_loss_backward", mutates_args=("gcols", "scratch", "gcoef"))
+def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor,
+                           r: torch.Tensor, weight: float, g_loss: torch.Tensor, gidx: List[int],
+                           gcols: List[torch.Tensor], scratch: torch.Tensor, gcoef: torch.Tensor,
+                           want_coef: bool) -> None:
+    n = r.numel()
+    ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward")
+    if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
+        raise FbpxError(f"residual_loss_backward: gradient columns {gidx} of {len(cols)}")
+    if coef.numel() != len(fa) or gcoef.numel() != len(fa) or scratch.numel() < _lib.FBPX_SUM_BLOCKS * len(fa):
+        raise FbpxError(f"residual_loss_backward: coef {coef.numel()} / gcoef {gcoef.numel()} for {len(fa)} "
+                        f"terms, scratch {scratch.numel()}")
+    gp = (ctypes.c_void_p * len(cols))()
+    for i, g in zip(gidx, gcols):
+        _p(g, torch.float32, "gcol")
+        if g.numel() != n:
+            raise FbpxError(f"residual_loss_backward: gradient column of {g.numel()} values for {n} points")
+        gp[i] = g.data_ptr()
+    check(lib().fbpx_residual_loss_backward(
+        _stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"), _p(r, torch.float32, "r"),
+        float(weight), _p(g_loss, torch.float32, "g_loss"), gp, _p(scratch, torch.float32, "scratch"),
+        _p(gcoef, torch.float32, "gcoef") if want_coef else None), "fbpx_residual_loss_backward")
+
+
+class ResidualLoss(torch.autograd.Function):
+    """weight / n * sum_k (sum_t coef[t] F(fa[t], k) F(fb[t], k))^2 over n
+    points (fbpx_residual_loss): cols are (n, 1) or (n,) float32 columns,
+    F(f, k) = cols[f][k] (1 for f < 0), coef an (n_terms,) float32 tensor.
+    Gradients for the columns and coefficients that require them."""
+
+    @staticmethod
+    def forward(ctx, coef, weight, fa, fb, *cols):
+        cs = [c.contiguous() for c in cols]
+        dev = cs[0].device
+        n = cs[0].numel()
+        r = torch.empty(n, dtype=torch.float32, device=dev)
+        scratch = torch.empty(_lib.FBPX_SUM_BLOCKS, dtype=torch.float32, device=dev)
+        loss = torch.empty((), dtype=torch.float32, device=dev)
+        residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss)
+        ctx.save_for_backward(coef, r, *cs)
+        ctx.weight, ctx.fa, ctx.fb = float(weight), list(fa), list(fb)
+        return loss
+
+    @staticmethod
+    def backward(ctx, g):
+        coef, r, *cs = ctx.saved_tensors
+        need = ctx.needs_input_grad
+        gidx = [i for i in range(len(cs)) if need[4 + i]]
+        gcols = [torch.empty_like(cs[i]) for i in gidx]
+        nt = len(ctx.fa)
+        gcoef = torch.empty(nt, dtype=torch.float32, device=r.device)
+        scratch = torch.empty(_lib.FBPX_SUM_BLOCKS * nt, dtype=torch.float32, device=r.device)
+        residual_loss_backward(cs, ctx.fa, ctx.fb, coef, r, ctx.weight, g.contiguous(), gidx, gcols, scratch,
+                               gcoef, bool(need[0]))
+        out = [None] * len(cs)
+        for i, gc in zip(gidx, gcols):
+            out[i] = gc
+        return (gcoef if need[0] else None, None, None, None, *out)
+
+
 class _Mean(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x):

@@ -227,6 +227,36 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
                               const float* B, int32_t n_req, const int32_t* req,
                               const float* const* gout, float* gq);
 
+/*
+ * Fused pointwise-residual loss, the form of the problems' loss_fn
+ * (problems.py:123-124, 204, 265, 326, 405: a weighted mean of squared
+ * pointwise residuals, replacing their chain of jnp ops and jnp.mean):
+ *   r[k]  = sum over t < n_terms of coef[t] * F(fa[t], k) * F(fb[t], k),
+ *   F(f, k) = col[f][k] for f >= 0, 1 for f < 0 (constant terms use fa = fb = -1),
+ *   loss  = weight / n * sum over k < n of r[k]^2.
+ * col: HOST array of n_cols <= FBPX_MAX_REQ device pointers to n floats (the
+ * constraint's jets and per-point data such as sources or targets); fa, fb:
+ * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
+ * on the device).  Writes r[n] and loss[0]; scratch: FBPX_SUM_BLOCKS floats.
+ * Deterministic (fixed-order reductions, no atomics, no memset).
+ */
+#define FBPX_MAX_TERMS 8
+#define FBPX_SUM_BLOCKS 256
+int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
+                       const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
+                       float* scratch, float* loss);
+
+/* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
+ * pointer to d L / d loss):
+ *   gcol[f][k] = s * r[k] * d r[k] / d col[f][k]   for every f with gcol[f] != NULL,
+ *   gcoef[t]   = s * sum over k of r[k] * (product of term t at k)   (gcoef may be NULL).
+ * gcol: HOST array of n_cols device pointers (NULL entries: per-point data,
+ * no gradient); scratch: FBPX_SUM_BLOCKS * n_terms floats. */
+int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                                int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
+                                const float* r, float weight, const float* g_loss, float* const* gcol,
+                                float* scratch, float* gcoef);
+
 /* ---------------------------------------------------------------- misc */
 
 /* Parameters per subdomain row for a network shape (0 if unsupported). */

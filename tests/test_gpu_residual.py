@@ -1,0 +1,136 @@
+"""GPU: the fused pointwise-residual loss (fbpx_residual_loss and its
+transpose, ops.ResidualLoss) against the problems' own loss_fn.
+
+* every built-in problem with a residual description: the engine's loss
+  epilogue with the fused kernels against the same epilogue running loss_fn,
+  on the same random jets (float32 both): loss, gradient with respect to the
+  jets of u~ and to trainable problem parameters within 1e-5 relative;
+* sizes around the reduction's block and grid edges up to 2^20 + 37 points:
+  the fused loss and its column gradients against loss_fn evaluated in
+  float64 on the CPU, 1e-5 relative;
+* the kernel's own term forms (constant term, a column squared, a column in
+  two terms, a column without gradient) against the float64 formula.
+"""
+
+import pytest
+import torch
+
+from fbpinns_amd import epilogue
+from fbpinns_amd import problems as P
+from fbpinns_amd.engine import pick_schema
+from fbpinns_amd.ops import NetSpec, ResidualLoss
+
+pytestmark = pytest.mark.gpu
+
+PHYS1 = ((0, ()), (0, (0,)), (0, (0, 0)))
+CASES = {
+    "ho_soft": (P.HarmonicOscillator1D, dict(d=2, w0=20), 1, [PHYS1, ((0, ()), (0, (0,)))], [0, 2]),
+    "ho_hard": (P.HarmonicOscillator1DHardBC, dict(d=2, w0=20), 1, [PHYS1], [0]),
+    "ho_inverse": (P.HarmonicOscillator1DInverse, dict(d=2, w0=20), 1, [PHYS1, ((0, ()),)], [0, 1]),
+    "burgers": (P.BurgersEquation2D, dict(), 2, [((0, ()), (0, (0,)), (0, (1,)), (0, (0, 0)))], [0]),
+    "wave": (P.WaveEquationConstantVelocity3D, dict(c0=1.3, source=P.wave_sources()), 3,
+             [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0]),
+    "wave_gauss": (P.WaveEquationGaussianVelocity3D, dict(c0=1.0, source=P.wave_sources()), 3,
+                   [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0]),
+    "poisson": (P.PoissonMultiscale2D, dict(), 2, [((0, (0, 0)), (0, (1, 1)))], [0]),
+}
+
+
+def _params(prob, kw, dev, trainable_value=0.7):
+    s, t = prob.init_params(**kw)
+    s = {k: (v.to(dev) if isinstance(v, torch.Tensor) else v) for k, v in s.items()}
+    t = {k: torch.full_like(v, trainable_value).to(dev).requires_grad_(True) for k, v in t.items()}
+    return {"static": {"problem": s}, "trainable": {"problem": t}}
+
+
+def _setup(name, sizes, dev, seed=0):
+    prob, kw, xd, reqs, nvals = CASES[name]
+    g = torch.Generator().manual_seed(seed)
+    cranges, off = [], 0
+    for s in sizes:
+        cranges.append((off, s))
+        off += s
+    x = (torch.rand((off, xd), generator=g) * 0.8 + 0.1).to(dev)
+    cvalues = []
+    for (o, s), nv in zip(cranges, nvals):
+        cvalues.append([x[o:o + s]] + [torch.randn((s, 1), generator=g).to(dev) for _ in range(nv)])
+    d1, d2 = pick_schema(xd, *epilogue.schema_masks(reqs, xd))
+    net = NetSpec(xd, 32, 2, d1, d2)
+    return prob, kw, x, cranges, cvalues, reqs, net
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_epilogue_matches_loss_fn(name):
+    dev = torch.device("cuda")
+    prob, kw, x, cranges, cvalues, reqs, net = _setup(name, [300, 1 if name == "ho_soft" else 13][:len(CASES[name][3])],
+                                                      dev)
+    ap = _params(prob, kw, dev)
+    ef = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, net)
+    eg = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, net, fused_loss=False)
+    assert ef.fused is not None and eg.fused is None
+    print(f"[residual] {name} residuals={len(ef.fused)} affine={ef.affine is not None}", flush=True)
+    q = (torch.randn((net.J, x.shape[0]), generator=torch.Generator().manual_seed(1)) * 0.3).to(dev)
+    q.requires_grad_(True)
+    leaves = [q] + list(ap["trainable"]["problem"].values())
+    lf = ef.loss(q, ap)
+    lg = eg.loss(q, ap)
+    e = abs(lf.item() - lg.item()) / abs(lg.item())
+    gf = torch.autograd.grad(lf, leaves)
+    gg = torch.autograd.grad(lg, leaves)
+    print(f"  loss fused {lf.item():.6e} loss_fn {lg.item():.6e} rel {e:.1e}", flush=True)
+    assert e < 1e-5
+    for a, b in zip(gf, gg):
+        eg_ = ((a - b).abs().max() / b.abs().max().clamp_min(1e-30)).item()
+        print(f"  grad {tuple(b.shape)} rel {eg_:.1e}", flush=True)
+        assert eg_ < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 65537, 1 << 20 | 37])
+def test_fused_loss_sizes_against_float64(n):
+    dev = torch.device("cuda")
+    prob, kw, x, cranges, cvalues, reqs, net = _setup("wave_gauss", [n], dev, seed=n)
+    ap = _params(prob, kw, dev)
+    ef = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, net)
+    assert ef.fused is not None
+    print(f"[residual] n={n}", flush=True)
+    g = torch.Generator().manual_seed(7)
+    jets64 = [torch.randn((n, 1), generator=g, dtype=torch.float64) for _ in reqs[0]]
+    jets = [j.float().to(dev).requires_grad_(True) for j in jets64]
+    lf = ef._fused_loss([jets], ap)
+    gf = torch.autograd.grad(lf, jets)
+    j64 = [j.clone().requires_grad_(True) for j in jets64]
+    ap64 = {"static": {"problem": {k: (v.cpu().double() if isinstance(v, torch.Tensor) else v)
+                                   for k, v in ap["static"]["problem"].items()}}, "trainable": {"problem": {}}}
+    l64 = prob.loss_fn(ap64, [[x.cpu().double()] + j64])
+    g64 = torch.autograd.grad(l64, j64)
+    e = abs(lf.item() - l64.item()) / abs(l64.item())
+    print(f"  loss rel {e:.1e}", flush=True)
+    assert e < 1e-5
+    for a, b in zip(gf, g64):
+        eg_ = ((a.cpu().double() - b).abs().max() / b.abs().max()).item()
+        assert eg_ < 1e-5, eg_
+
+
+def test_residual_term_forms():
+    """r = 2 a^2 - 0.5 a b + 3 + b c, loss = 0.25 mean r^2, c without gradient."""
+    dev = torch.device("cuda")
+    n = 5000
+    g = torch.Generator().manual_seed(3)
+    cols64 = [torch.randn(n, generator=g, dtype=torch.float64) for _ in range(3)]
+    coef64 = torch.tensor([2.0, -0.5, 3.0, 1.0], dtype=torch.float64)
+    fa, fb = [0, 0, -1, 1], [0, 1, -1, 2]
+    cols = [c.float().to(dev) for c in cols64]
+    cols[0].requires_grad_(True)
+    cols[1].requires_grad_(True)
+    coef = coef64.float().to(dev).requires_grad_(True)
+    lf = ResidualLoss.apply(coef, 0.25, fa, fb, *cols)
+    ga, gb, gc = torch.autograd.grad(lf, [cols[0], cols[1], coef])
+    c64 = [c.clone().requires_grad_(i < 2) for i, c in enumerate(cols64)]
+    k64 = coef64.clone().requires_grad_(True)
+    a, b, c = c64
+    r = k64[0] * a * a + k64[1] * a * b + k64[2] + k64[3] * b * c
+    l64 = 0.25 * (r ** 2).mean()
+    ra, rb, rc = torch.autograd.grad(l64, [a, b, k64])
+    assert abs(lf.item() - l64.item()) / l64.item() < 1e-5
+    for u, v in ((ga, ra), (gb, rb), (gc, rc)):
+        assert ((u.cpu().double() - v).abs().max() / v.abs().max()).item() < 1e-5
