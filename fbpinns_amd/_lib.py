@@ -15,7 +15,7 @@ FBPX_MAX_XD = 3
 FBPX_CHUNK = 512
 FBPX_MAX_REQ = 16
 FBPX_MAX_TERMS = 8
-FBPX_SUM_BLOCKS = 256
+FBPX_RESIDUAL_BLOCKS = 1024
 
 _c_i32 = ctypes.c_int32
 _c_p = ctypes.c_void_p
@@ -70,6 +70,7 @@ SIGNATURES = {
     "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
                                              ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, _c_p,
                                              ctypes.c_float, _c_p, ctypes.POINTER(_c_p), _c_p, _c_p]),
+    "fbpx_segment_sum": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_last_error": (ctypes.c_char_p, []),

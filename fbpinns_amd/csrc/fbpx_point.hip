@@ -415,9 +415,28 @@ struct ResidualArgs {
   int dterm[2 * FBPX_MAX_TERMS], dother[2 * FBPX_MAX_TERMS];
 };
 
-__device__ inline float residual_term(const ResidualArgs& a, int t, int64_t k) {
-  const float x = a.fa[t] >= 0 ? a.col[a.fa[t]][k] : 1.f;
-  return a.fb[t] >= 0 ? x * a.col[a.fb[t]][k] : x;
+// Points per thread and loop trip: independent loads in flight per thread.
+constexpr int kResUnroll = 4;
+
+// Term t's factor columns (nullptr: factor 1), uniform across the grid.
+struct TermCols {
+  const float* pa[FBPX_MAX_TERMS];
+  const float* pb[FBPX_MAX_TERMS];
+};
+
+__device__ inline TermCols term_cols(const ResidualArgs& a) {
+  TermCols c;
+#pragma unroll
+  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
+    c.pa[t] = t < a.n_terms && a.fa[t] >= 0 ? a.col[a.fa[t]] : nullptr;
+    c.pb[t] = t < a.n_terms && a.fb[t] >= 0 ? a.col[a.fb[t]] : nullptr;
+  }
+  return c;
+}
+
+__device__ inline float term_at(const TermCols& c, int t, int64_t k) {
+  const float x = c.pa[t] ? c.pa[t][k] : 1.f;
+  return c.pb[t] ? x * c.pb[t][k] : x;
 }
 
 // r[k] = sum_t coef[t] term_t(k); scratch[block] = sum of r^2 over the
@@ -425,14 +444,32 @@ __device__ inline float residual_term(const ResidualArgs& a, int t, int64_t k) {
 __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
                                                        float* __restrict__ r, float* __restrict__ scratch) {
   __shared__ float sm[256];
-  float s = 0.f;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-    float v = 0.f;
+  const TermCols c = term_cols(a);
+  float cf[FBPX_MAX_TERMS];
 #pragma unroll
-    for (int t = 0; t < FBPX_MAX_TERMS; ++t)
-      if (t < a.n_terms) v = fmaf(coef[t], residual_term(a, t, k), v);
-    r[k] = v;
-    s = fmaf(v, v, s);
+  for (int t = 0; t < FBPX_MAX_TERMS; ++t) cf[t] = t < a.n_terms ? coef[t] : 0.f;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float s = 0.f;
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
+    float v[kResUnroll];
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      v[u] = 0.f;
+      if (k < n) {
+#pragma unroll
+        for (int t = 0; t < FBPX_MAX_TERMS; ++t)
+          if (t < a.n_terms) v[u] = fmaf(cf[t], term_at(c, t, k), v[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      if (k < n) {
+        r[k] = v[u];
+        s = fmaf(v[u], v[u], s);
+      }
+    }
   }
   const float tot = block_tree_sum(s, sm);
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
@@ -446,24 +483,45 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
                                                                 const float* __restrict__ g_loss,
                                                                 float* __restrict__ scratch) {
   __shared__ float sm[256];
+  const TermCols c = term_cols(a);
   const float sc = g_loss[0] * w2n;
-  float acc[FBPX_MAX_TERMS];
+  float cf[FBPX_MAX_TERMS], acc[FBPX_MAX_TERMS];
 #pragma unroll
-  for (int t = 0; t < FBPX_MAX_TERMS; ++t) acc[t] = 0.f;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-    const float sr = sc * r[k];
+  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
+    cf[t] = t < a.n_terms ? coef[t] : 0.f;
+    acc[t] = 0.f;
+  }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
+    float sr[kResUnroll];
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      sr[u] = k < n ? sc * r[k] : 0.f;
+    }
     for (int f = 0; f < a.n_cols; ++f) {
-      if (a.gcol[f] == nullptr) continue;
-      float g = 0.f;
-      for (int e = a.dstart[f]; e < a.dstart[f + 1]; ++e) {
-        const float other = a.dother[e] >= 0 ? a.col[a.dother[e]][k] : 1.f;
-        g = fmaf(coef[a.dterm[e]], other, g);
+      float* gc = a.gcol[f];
+      if (gc == nullptr) continue;
+#pragma unroll
+      for (int u = 0; u < kResUnroll; ++u) {
+        const int64_t k = k0 + u * stride;
+        if (k >= n) continue;
+        float g = 0.f;
+        for (int e = a.dstart[f]; e < a.dstart[f + 1]; ++e) {
+          const float other = a.dother[e] >= 0 ? a.col[a.dother[e]][k] : 1.f;
+          g = fmaf(cf[a.dterm[e]], other, g);
+        }
+        gc[k] = sr[u] * g;
       }
-      a.gcol[f][k] = sr * g;
     }
 #pragma unroll
-    for (int t = 0; t < FBPX_MAX_TERMS; ++t)
-      if (t < a.n_terms) acc[t] = fmaf(sr, residual_term(a, t, k), acc[t]);
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      if (k >= n) continue;
+#pragma unroll
+      for (int t = 0; t < FBPX_MAX_TERMS; ++t)
+        if (t < a.n_terms) acc[t] = fmaf(sr[u], term_at(c, t, k), acc[t]);
+    }
   }
 #pragma unroll
   for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
@@ -474,16 +532,32 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
   }
 }
 
-// out[j] = scale * sum over b < nb of scratch[b * ncol + j], j < ncol (one block).
+// out[j] = scale * sum over b < nb of scratch[b * ncol + j], j < ncol (one
+// block; thread i sums b = i, i + 256, ... in order, then a tree).
 __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restrict__ scratch, int nb, int ncol,
                                                           float scale, float* __restrict__ out) {
   __shared__ float sm[256];
   for (int j = 0; j < ncol; ++j) {
-    const float v = (int)threadIdx.x < nb ? scratch[threadIdx.x * ncol + j] : 0.f;
+    float v = 0.f;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += scratch[b * ncol + j];
     const float t = block_tree_sum(v, sm);
     if (threadIdx.x == 0) out[j] = scale * t;
     __syncthreads();
   }
+}
+
+// out[s] = sum over j in [seg_start[s], seg_start[s + 1]) of x[perm[j]] (or
+// x[j]); one block per segment, fixed order.
+__global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restrict__ seg_start,
+                                                          const int32_t* __restrict__ perm,
+                                                          const float* __restrict__ x, float* __restrict__ out) {
+  __shared__ float sm[256];
+  const int s = blockIdx.x;
+  const int j0 = seg_start[s], j1 = seg_start[s + 1];
+  float v = 0.f;
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) v += x[perm ? perm[j] : j];
+  const float t = block_tree_sum(v, sm);
+  if (threadIdx.x == 0) out[s] = t;
 }
 
 __global__ void count_increment_kernel(int32_t* count) {
@@ -732,6 +806,11 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
   return hip_check(hipGetLastError(), "affine_jets_backward");
 }
 
+static int residual_blocks(int64_t n) {
+  const int64_t need = (n + 256 * kResUnroll - 1) / (256 * kResUnroll);
+  return (int)(need < FBPX_RESIDUAL_BLOCKS ? need : FBPX_RESIDUAL_BLOCKS);
+}
+
 static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms, const int32_t* fa,
                          const int32_t* fb, float* const* gcol, ResidualArgs* a) {
   if (n <= 0 || n_cols < 1 || n_cols > FBPX_MAX_REQ || n_terms < 1 || n_terms > FBPX_MAX_TERMS || !col || !fa ||
@@ -788,8 +867,7 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
     return FBPX_EINVAL;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int64_t need = (n + 255) / 256;
-  const int nb = (int)(need < kSumBlocks ? need : kSumBlocks);
+  const int nb = residual_blocks(n);
   hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch);
   hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, weight / (float)n, loss);
   return hip_check(hipGetLastError(), "residual_loss");
@@ -807,12 +885,22 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
     return FBPX_EINVAL;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int64_t need = (n + 255) / 256;
-  const int nb = (int)(need < kSumBlocks ? need : kSumBlocks);
+  const int nb = residual_blocks(n);
   hipLaunchKernelGGL(residual_backward_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, 2.f * weight / (float)n,
                      g_loss, scratch);
   if (gcoef) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
   return hip_check(hipGetLastError(), "residual_loss_backward");
+}
+
+int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
+                     float* out) {
+  if (n_seg < 0 || (n_seg > 0 && (!seg_start || !x || !out))) {
+    set_error("segment_sum: n_seg=%d, null seg_start / x / out", n_seg);
+    return FBPX_EINVAL;
+  }
+  if (n_seg == 0) return FBPX_OK;
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(256), 0, (hipStream_t)stream, seg_start, perm, x, out);
+  return hip_check(hipGetLastError(), "segment_sum");
 }
 
 int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out) {

@@ -28,8 +28,8 @@ import torch
 from torch.func import jvp
 
 from fbpinns_amd._lib import FBPX_MAX_REQ
-from fbpinns_amd.ops import AffineJets, FbpxError, ResidualLoss
-from fbpinns_amd.residuals import F, J, V, check_spec, spec_loss
+from fbpinns_amd.ops import AffineJets, FbpxError, GatherRows, ResidualLoss
+from fbpinns_amd.residuals import F, J, P, V, check_spec, spec_loss
 from fbpinns_amd.util.logger import logger
 
 
@@ -210,9 +210,14 @@ class Epilogue:
             if not self._fused_agrees(spec, all_params, n_values):
                 logger.info("fused residual loss: description disagrees with loss_fn, using loss_fn")
                 return None
+            return self._fused_plans(spec, all_params)
         except Exception as e:  # noqa: BLE001 - any failure selects loss_fn
-            logger.info(f"fused residual loss not used: {e}")
+            logger.warning(f"fused residual loss not used ({type(e).__name__}: {e}); running loss_fn")
             return None
+
+    def _fused_plans(self, spec, all_params):
+        """Per residual: (constraint, weight, column keys, per-point data,
+        factor columns fa / fb, static coefficients, trainable coefficients)."""
         plans = []
         for res in spec:
             ic = res.constraint
@@ -223,16 +228,33 @@ class Epilogue:
             def slot(f):
                 if f is None:
                     return -1
-                key = ("J", f.i) if isinstance(f, J) else ("V", f.j) if isinstance(f, V) else ("F", id(f))
+                key = ("J", f.i) if isinstance(f, J) else ("V", f.j) if isinstance(f, V) else \
+                    ("P", id(f)) if isinstance(f, P) else ("F", id(f))
                 if key not in keys:
                     keys.append(key)
-                    if isinstance(f, V):
+                    vals = [v.double() if v.is_floating_point() else v for v in self.cvalues[ic][1:]]
+                    if isinstance(f, J):
+                        pass
+                    elif isinstance(f, V):
                         data[key] = self.cvalues[ic][1 + f.j].to(torch.float32).reshape(n_c).contiguous()
                     elif isinstance(f, F):
                         with torch.no_grad():
-                            v = f.fn(all_params, x_c.double())
+                            v = f.fn(all_params, x_c.double(), *vals)
                         data[key] = torch.as_tensor(v).to(torch.float64).expand(n_c, 1).reshape(n_c) \
                             .to(torch.float32).contiguous()
+                    else:
+                        # static index: points sorted by index once, for the
+                        # deterministic transpose of the gather
+                        idx = self.cvalues[ic][1 + f.j].reshape(n_c).long()
+                        n_vec = f.fn(all_params).numel()
+                        if idx.min() < 0 or idx.max() >= n_vec:
+                            raise ValueError(f"gather index outside [0, {n_vec})")
+                        perm = torch.argsort(idx, stable=True).to(torch.int32)
+                        counts = torch.bincount(idx, minlength=n_vec)
+                        seg = torch.zeros(n_vec + 1, dtype=torch.int64, device=idx.device)
+                        seg[1:] = torch.cumsum(counts, 0)
+                        data[key] = (f.fn, idx.to(torch.int32).contiguous(), perm.contiguous(),
+                                     seg.to(torch.int32).contiguous())
                 return keys.index(key)
             for _, a, b in res.terms:
                 fa.append(slot(a))
@@ -277,7 +299,16 @@ class Epilogue:
     def _fused_loss(self, ujss, all_params):
         loss = None
         for ic, weight, keys, data, fa, fb, static, dyn in self.fused:
-            cols = [ujss[ic][k[1]].reshape(-1) if k[0] == "J" else data[k] for k in keys]
+            cols = []
+            for k in keys:
+                if k[0] == "J":
+                    cols.append(ujss[ic][k[1]].reshape(-1))
+                elif k[0] == "P":
+                    fn, idx, perm, seg = data[k]
+                    v = torch.as_tensor(fn(all_params)).reshape(-1).to(torch.float32)
+                    cols.append(GatherRows.apply(v, idx, perm, seg))
+                else:
+                    cols.append(data[k])
             coef = static
             if dyn:
                 parts = list(static.unbind())

@@ -333,7 +333,7 @@ def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: 
                   r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor) -> None:
     n = r.numel()
     ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss")
-    if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_SUM_BLOCKS or loss.numel() != 1:
+    if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS or loss.numel() != 1:
         raise FbpxError(f"residual_loss: coef {coef.numel()} for {len(fa)} terms, scratch {scratch.numel()}, "
                         f"loss {loss.numel()}")
     check(lib().fbpx_residual_loss(_stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
@@ -350,7 +350,7 @@ def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int
     ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward")
     if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
         raise FbpxError(f"residual_loss_backward: gradient columns {gidx} of {len(cols)}")
-    if coef.numel() != len(fa) or gcoef.numel() != len(fa) or scratch.numel() < _lib.FBPX_SUM_BLOCKS * len(fa):
+    if coef.numel() != len(fa) or gcoef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS * len(fa):
         raise FbpxError(f"residual_loss_backward: coef {coef.numel()} / gcoef {gcoef.numel()} for {len(fa)} "
                         f"terms, scratch {scratch.numel()}")
     gp = (ctypes.c_void_p * len(cols))()
@@ -377,7 +377,7 @@ class ResidualLoss(torch.autograd.Function):
         dev = cs[0].device
         n = cs[0].numel()
         r = torch.empty(n, dtype=torch.float32, device=dev)
-        scratch = torch.empty(_lib.FBPX_SUM_BLOCKS, dtype=torch.float32, device=dev)
+        scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=dev)
         loss = torch.empty((), dtype=torch.float32, device=dev)
         residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss)
         ctx.save_for_backward(coef, r, *cs)
@@ -392,13 +392,43 @@ class ResidualLoss(torch.autograd.Function):
         gcols = [torch.empty_like(cs[i]) for i in gidx]
         nt = len(ctx.fa)
         gcoef = torch.empty(nt, dtype=torch.float32, device=r.device)
-        scratch = torch.empty(_lib.FBPX_SUM_BLOCKS * nt, dtype=torch.float32, device=r.device)
+        scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=r.device)
         residual_loss_backward(cs, ctx.fa, ctx.fb, coef, r, ctx.weight, g.contiguous(), gidx, gcols, scratch,
                                gcoef, bool(need[0]))
         out = [None] * len(cs)
         for i, gc in zip(gidx, gcols):
             out[i] = gc
         return (gcoef if need[0] else None, None, None, None, *out)
+
+
+@torch.library.custom_op("fbpx::segment_sum", mutates_args=("out",))
+def segment_sum(seg_start: torch.Tensor, perm: torch.Tensor, x: torch.Tensor, out: torch.Tensor) -> None:
+    n_seg = out.numel()
+    if seg_start.numel() != n_seg + 1 or perm.numel() != x.numel():
+        raise FbpxError(f"segment_sum: {seg_start.numel()} offsets for {n_seg} segments, "
+                        f"{perm.numel()} permutation entries for {x.numel()} values")
+    check(lib().fbpx_segment_sum(_stream(), n_seg, _p(seg_start, torch.int32, "seg_start"),
+                                 _p(perm, torch.int32, "perm"), _p(x, torch.float32, "x"),
+                                 _p(out, torch.float32, "out")), "fbpx_segment_sum")
+
+
+class GatherRows(torch.autograd.Function):
+    """y[k] = v[idx[k]] for a trainable vector v; the backward sums the
+    incoming gradient per index with fbpx_segment_sum over the points sorted
+    by index (perm, seg_start: precomputed, the index is static)."""
+
+    @staticmethod
+    def forward(ctx, v, idx, perm, seg_start):
+        ctx.save_for_backward(perm, seg_start)
+        ctx.n = v.numel()
+        return v.reshape(-1).index_select(0, idx)
+
+    @staticmethod
+    def backward(ctx, g):
+        perm, seg_start = ctx.saved_tensors
+        out = torch.empty(ctx.n, dtype=torch.float32, device=g.device)
+        segment_sum(seg_start, perm, g.contiguous(), out)
+        return out, None, None, None
 
 
 class _Mean(torch.autograd.Function):

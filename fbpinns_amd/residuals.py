@@ -15,9 +15,11 @@ where each term is (coef, a, b):
         coefficients such as a learnt damping);
   a, b  None (factor 1), J(i) (the i-th derivative the constraint requests,
         in its required_ujs order), V(j) (the j-th constraining value of the
-        constraint, a per-point (n, 1) tensor) or F(fn) (per-point data
-        fn(all_params, x_c) -> (n, 1) or (1, 1), from static parameters only:
-        it is evaluated once per work list).
+        constraint, a per-point (n, 1) tensor), F(fn) (per-point data
+        fn(all_params, x_c, *values_c) -> (n, 1) or (1, 1), from static
+        parameters only: it is evaluated once per work list) or P(fn, j) (a
+        trainable vector fn(all_params) gathered per point by the j-th
+        constraining value, an integer index: per-instance coefficients).
 
 The engine then evaluates each residual with one fused kernel pair
 (fbpx_residual_loss, and its transpose in the backward) instead of the chain
@@ -53,7 +55,14 @@ class F:
     fn: Callable
 
 
-Factor = Optional[Union[J, V, F]]
+@dataclass(frozen=True)
+class P:
+    """Trainable vector fn(all_params) gathered by the j-th constraining value."""
+    fn: Callable
+    j: int
+
+
+Factor = Optional[Union[J, V, F, P]]
 Coef = Union[float, Callable]
 
 
@@ -84,7 +93,7 @@ def check_spec(spec, n_constraints, n_req, n_values):
                     continue
                 if isinstance(f, J) and 0 <= f.i < n_req[ic]:
                     continue
-                if isinstance(f, V) and 0 <= f.j < n_values[ic]:
+                if isinstance(f, (V, P)) and 0 <= f.j < n_values[ic]:
                     continue
                 raise ValueError(f"factor {f!r} outside constraint {ic} "
                                  f"({n_req[ic]} derivatives, {n_values[ic]} values)")
@@ -108,7 +117,9 @@ def spec_loss(spec, all_params, constraints, n_values):
                 return c[1 + nv + f.i]
             if isinstance(f, V):
                 return c[1 + f.j]
-            return f.fn(all_params, x)
+            if isinstance(f, P):
+                return f.fn(all_params).reshape(-1)[c[1 + f.j].reshape(-1).long()].reshape(-1, 1)
+            return f.fn(all_params, x, *c[1:1 + nv])
         r = 0.0
         for coef, a, b in res.terms:
             cf = coef(all_params) if callable(coef) else coef

@@ -237,11 +237,11 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * col: HOST array of n_cols <= FBPX_MAX_REQ device pointers to n floats (the
  * constraint's jets and per-point data such as sources or targets); fa, fb:
  * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
- * on the device).  Writes r[n] and loss[0]; scratch: FBPX_SUM_BLOCKS floats.
+ * on the device).  Writes r[n] and loss[0]; scratch: FBPX_RESIDUAL_BLOCKS floats.
  * Deterministic (fixed-order reductions, no atomics, no memset).
  */
 #define FBPX_MAX_TERMS 8
-#define FBPX_SUM_BLOCKS 256
+#define FBPX_RESIDUAL_BLOCKS 1024
 int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
                        const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
                        float* scratch, float* loss);
@@ -251,11 +251,24 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
  *   gcol[f][k] = s * r[k] * d r[k] / d col[f][k]   for every f with gcol[f] != NULL,
  *   gcoef[t]   = s * sum over k of r[k] * (product of term t at k)   (gcoef may be NULL).
  * gcol: HOST array of n_cols device pointers (NULL entries: per-point data,
- * no gradient); scratch: FBPX_SUM_BLOCKS * n_terms floats. */
+ * no gradient); scratch: FBPX_RESIDUAL_BLOCKS * n_terms floats. */
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
                                 int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
                                 const float* r, float weight, const float* g_loss, float* const* gcol,
                                 float* scratch, float* gcoef);
+
+/*
+ * Deterministic segmented sum, the transpose of a per-point gather
+ * y[k] = v[idx[k]] of a trainable vector (a per-instance coefficient such as
+ * each oscillator's learnt damping; the reverse mode of jax's gather,
+ * a scatter-add, which the reference would run for such a loss):
+ *   out[s] = sum over j in [seg_start[s], seg_start[s+1]) of x[perm[j]]
+ * (perm NULL: x[j]) for s < n_seg, in increasing j, no atomics.  With perm a
+ * stable argsort of idx and seg_start its segment offsets, out = gather^T x.
+ * seg_start: DEVICE, n_seg + 1 int32; perm: DEVICE int32 or NULL.
+ */
+int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
+                     float* out);
 
 /* ---------------------------------------------------------------- misc */
 

@@ -9,7 +9,10 @@ transpose, ops.ResidualLoss) against the problems' own loss_fn.
   the fused loss and its column gradients against loss_fn evaluated in
   float64 on the CPU, 1e-5 relative;
 * the kernel's own term forms (constant term, a column squared, a column in
-  two terms, a column without gradient) against the float64 formula.
+  two terms, a column without gradient) against the float64 formula;
+* the per-point gather of a trainable vector and its transpose
+  (fbpx_segment_sum) against float64 index_add, with unsorted indices and
+  empty segments.
 """
 
 import pytest
@@ -18,7 +21,7 @@ import torch
 from fbpinns_amd import epilogue
 from fbpinns_amd import problems as P
 from fbpinns_amd.engine import pick_schema
-from fbpinns_amd.ops import NetSpec, ResidualLoss
+from fbpinns_amd.ops import GatherRows, NetSpec, ResidualLoss
 
 pytestmark = pytest.mark.gpu
 
@@ -33,13 +36,24 @@ CASES = {
     "wave_gauss": (P.WaveEquationGaussianVelocity3D, dict(c0=1.0, source=P.wave_sources()), 3,
                    [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0]),
     "poisson": (P.PoissonMultiscale2D, dict(), 2, [((0, (0, 0)), (0, (1, 1)))], [0]),
+    "batched_inverse": (P.BatchedHarmonicOscillatorInverse,
+                        dict(w0s=[20.0, 35.0, 50.0, 80.0, 90.0], mus_true=[1.0, 2.0, 4.0, 6.0, 3.0], n_obs=4), 1,
+                        [PHYS1, ((0, ()),)], [["i"], ["f", "i"]]),
 }
+N_INST = 5
+
+
+def _value(kind, n, g):
+    if kind == "i":
+        return torch.randint(0, N_INST, (n,), generator=g)
+    return torch.randn((n, 1), generator=g)
 
 
 def _params(prob, kw, dev, trainable_value=0.7):
     s, t = prob.init_params(**kw)
     s = {k: (v.to(dev) if isinstance(v, torch.Tensor) else v) for k, v in s.items()}
-    t = {k: torch.full_like(v, trainable_value).to(dev).requires_grad_(True) for k, v in t.items()}
+    t = {k: (trainable_value + 0.1 * torch.arange(v.numel(), dtype=v.dtype).reshape(v.shape)).to(dev)
+         .requires_grad_(True) for k, v in t.items()}
     return {"static": {"problem": s}, "trainable": {"problem": t}}
 
 
@@ -53,7 +67,8 @@ def _setup(name, sizes, dev, seed=0):
     x = (torch.rand((off, xd), generator=g) * 0.8 + 0.1).to(dev)
     cvalues = []
     for (o, s), nv in zip(cranges, nvals):
-        cvalues.append([x[o:o + s]] + [torch.randn((s, 1), generator=g).to(dev) for _ in range(nv)])
+        kinds = ["f"] * nv if isinstance(nv, int) else nv
+        cvalues.append([x[o:o + s]] + [_value(k, s, g).to(dev) for k in kinds])
     d1, d2 = pick_schema(xd, *epilogue.schema_masks(reqs, xd))
     net = NetSpec(xd, 32, 2, d1, d2)
     return prob, kw, x, cranges, cvalues, reqs, net
@@ -62,8 +77,8 @@ def _setup(name, sizes, dev, seed=0):
 @pytest.mark.parametrize("name", list(CASES))
 def test_fused_epilogue_matches_loss_fn(name):
     dev = torch.device("cuda")
-    prob, kw, x, cranges, cvalues, reqs, net = _setup(name, [300, 1 if name == "ho_soft" else 13][:len(CASES[name][3])],
-                                                      dev)
+    sizes = [300, 1 if name == "ho_soft" else 13][:len(CASES[name][3])]
+    prob, kw, x, cranges, cvalues, reqs, net = _setup(name, sizes, dev)
     ap = _params(prob, kw, dev)
     ef = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, net)
     eg = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, net, fused_loss=False)
@@ -134,3 +149,24 @@ def test_residual_term_forms():
     assert abs(lf.item() - l64.item()) / l64.item() < 1e-5
     for u, v in ((ga, ra), (gb, rb), (gc, rc)):
         assert ((u.cpu().double() - v).abs().max() / v.abs().max()).item() < 1e-5
+
+
+@pytest.mark.parametrize("n,n_vec", [(1, 1), (1000, 7), (70000, 300)])
+def test_gather_transpose_matches_index_add(n, n_vec):
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(n)
+    idx = torch.randint(0, max(n_vec - 2, 1), (n,), generator=g)      # the last indices stay empty
+    v64 = torch.randn(n_vec, generator=g, dtype=torch.float64)
+    gy64 = torch.randn(n, generator=g, dtype=torch.float64)
+    perm = torch.argsort(idx, stable=True).to(torch.int32).to(dev)
+    seg = torch.zeros(n_vec + 1, dtype=torch.int64)
+    seg[1:] = torch.cumsum(torch.bincount(idx, minlength=n_vec), 0)
+    v = v64.float().to(dev).requires_grad_(True)
+    y = GatherRows.apply(v, idx.to(torch.int32).to(dev), perm, seg.to(torch.int32).to(dev))
+    assert torch.equal(y.detach().cpu(), v64.float()[idx])
+    (gv,) = torch.autograd.grad(y, v, gy64.float().to(dev))
+    ref = torch.zeros(n_vec, dtype=torch.float64).index_add_(0, idx, gy64)
+    e = ((gv.cpu().double() - ref).abs().max() / ref.abs().max().clamp_min(1e-30)).item()
+    print(f"[gather] n={n} n_vec={n_vec} rel {e:.1e}", flush=True)
+    assert e < 1e-5
+    assert (gv[n_vec - 2:].cpu() == 0).all() if n_vec > 2 else True

@@ -17,6 +17,7 @@ from fbpinns_amd import problems as P
 from fbpinns_amd.engine import pick_schema
 from fbpinns_amd.ops import NetSpec
 from fbpinns_amd.residuals import J, Residual, V, check_spec
+from fbpinns_amd.residuals import P as PF
 
 PHYS1 = ((0, ()), (0, (0,)), (0, (0, 0)))
 CASES = {
@@ -29,20 +30,31 @@ CASES = {
     "wave_gauss": (P.WaveEquationGaussianVelocity3D, dict(c0=1.0, source=P.wave_sources()), 3,
                    [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0], [50]),
     "poisson": (P.PoissonMultiscale2D, dict(), 2, [((0, (0, 0)), (0, (1, 1)))], [0], [50]),
+    "batched_inverse": (P.BatchedHarmonicOscillatorInverse,
+                        dict(w0s=[20.0, 35.0, 50.0, 80.0], mus_true=[1.0, 2.0, 4.0, 6.0], n_obs=4), 1,
+                        [PHYS1, ((0, ()),)], [["i"], ["f", "i"]], [40, 16]),
 }
+
+
+def _value(kind, n, g, n_inst=4):
+    if kind == "i":
+        return torch.randint(0, n_inst, (n,), generator=g)
+    return torch.randn((n, 1), generator=g, dtype=torch.float64)
 
 
 def _setup(name):
     prob, kw, xd, reqs, nvals, sizes = CASES[name]
     s, t = prob.init_params(**kw)
-    ap = {"static": {"problem": s}, "trainable": {"problem": {k: torch.full_like(v, 0.7) for k, v in t.items()}}}
+    ap = {"static": {"problem": s},
+          "trainable": {"problem": {k: 0.7 + 0.1 * torch.arange(v.numel(), dtype=v.dtype).reshape(v.shape)
+                                    for k, v in t.items()}}}
     g = torch.Generator().manual_seed(0)
     cranges, off = [], 0
     for n in sizes:
         cranges.append((off, n))
         off += n
     x = torch.rand((off, xd), generator=g, dtype=torch.float64) * 0.8 + 0.1
-    cvalues = [[x[o:o + n]] + [torch.randn((n, 1), generator=g, dtype=torch.float64) for _ in range(nv)]
+    cvalues = [[x[o:o + n]] + [_value(k, n, g) for k in (["f"] * nv if isinstance(nv, int) else nv)]
                for (o, n), nv in zip(cranges, nvals)]
     d1, d2 = pick_schema(xd, *epilogue.schema_masks(reqs, xd))
     ep = epilogue.Epilogue(prob, ap, x, cranges, cvalues, reqs, NetSpec(xd, 32, 2, d1, d2))
@@ -68,6 +80,10 @@ def _mutations(spec, n_req):
             else:
                 terms[it] = (-c if c != 0 else 1.0, a, b)
             out.append(("coef", ir, it, terms))
+            if isinstance(a, PF):
+                terms = list(res.terms)
+                terms[it] = (c, PF(lambda p, fn=a.fn: torch.roll(fn(p), 1), a.j), b)
+                out.append(("gather", ir, it, terms))
             if isinstance(a, J) and n_req[res.constraint] > 1:
                 terms = list(res.terms)
                 terms[it] = (c, J((a.i + 1) % n_req[res.constraint]), b)
@@ -114,5 +130,38 @@ def test_check_spec_rejects_bad_structure():
 
 def test_problem_without_description_runs_loss_fn():
     assert P.Problem.residuals({}) is None
-    assert not hasattr(P.BatchedHarmonicOscillatorInverse, "residuals") or \
-        P.BatchedHarmonicOscillatorInverse.residuals({}) is None
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_plans_structure(name):
+    """Per residual: every term factor names a column slot, derivative slots
+    index the constraint's requests, per-point data columns are float32 (n,),
+    gathers carry a permutation that sorts their index and segment offsets
+    that count it."""
+    prob, ap, ep, nvals, reqs = _setup(name)
+    spec = prob.residuals(ap)
+    plans = ep._fused_plans(spec, ap)
+    assert len(plans) == len(spec)
+    for (ic, weight, keys, data, fa, fb, static, dyn), res in zip(plans, spec):
+        n_c = ep.cranges[ic][1]
+        assert ic == res.constraint and weight == res.weight
+        assert len(fa) == len(fb) == len(res.terms) == static.numel()
+        assert len(set(keys)) == len(keys)
+        for f in fa + fb:
+            assert -1 <= f < len(keys)
+        for k in keys:
+            if k[0] == "J":
+                assert 0 <= k[1] < len(reqs[ic])
+            elif k[0] == "P":
+                fn, idx, perm, seg = data[k]
+                assert idx.dtype == perm.dtype == seg.dtype == torch.int32
+                srt = idx[perm.long()]
+                assert (srt[1:] >= srt[:-1]).all()
+                assert int(seg[-1]) == n_c and torch.equal(seg[1:] - seg[:-1], torch.bincount(
+                    idx.long(), minlength=seg.numel() - 1).to(torch.int32))
+            else:
+                assert data[k].dtype == torch.float32 and data[k].shape == (n_c,)
+        for t, (c, a, b) in enumerate(res.terms):
+            assert (t in [d[0] for d in dyn]) == callable(c)
+            if not callable(c):
+                assert float(static[t]) == pytest.approx(float(c), rel=1e-7)
