@@ -257,9 +257,19 @@ def main():
             pl.step()
         torch.cuda.synchronize()
         pdt = (time.time() - t0) / a.plain_steps
+        # matched solutions: the fused step from the same start, as many
+        # steps, then both models on the test grid
+        for _ in range(a.plain_steps):
+            eng2.step()
+        xt = x_test.to(eng2.device, torch.float32).contiguous()
+        u_f, u_p = eng2.model(xt), pl.model(xt)
+        u_diff = ((u_f - u_p).abs().mean() / u_p.abs().mean()).item()
         plain = {"steps_per_s": 1.0 / pdt, "ms_per_step": pdt * 1e3, "point_chunks": pl.point_chunks,
-                 "first_step_loss_rel_diff_vs_fused": abs(l_plain0 - l_fused0) / abs(l_fused0)}
-        log(f"plain: {1.0 / pdt:.4f} steps/s; speed-up {sps * pdt:.1f}x")
+                 "first_step_loss_rel_diff_vs_fused": abs(l_plain0 - l_fused0) / abs(l_fused0),
+                 "matched_after_steps": 1 + a.plain_steps, "test_points": int(xt.shape[0]),
+                 "test_u_rel_l1_diff_vs_fused": u_diff}
+        log(f"plain: {1.0 / pdt:.4f} steps/s; speed-up {sps * pdt:.1f}x; after {1 + a.plain_steps} steps "
+            f"mean |u_fused - u_plain| / mean |u_plain| = {u_diff:.2e} on {xt.shape[0]} test points")
         del pl, eng2
         torch.cuda.empty_cache()
 
