@@ -813,6 +813,16 @@ struct DW2Own {
   static constexpr int WS = 4 * PG;                 // per wave (16 pairs)
   static constexpr int BS = 4 * WS;                 // per 16-unit block (4 waves)
   static constexpr int TSZ = NRB * BS;              // floats per transpose buffer
+  // float offset of pair 4 gp + pp, units 4 cu .. 4 cu + 3, in a wave's WS
+  // floats.  SWZ: the 16-byte chunk c = 4 pp + cu is stored at
+  // c ^ (gp ^ (pp & 2)) within its 64-float group, so that the 128-bit
+  // stores of the 16 pairs do not collide in the banks (the unpadded h = 64
+  // groups alias); the reads of a pair group stay 64 distinct floats
+  template <bool SWZ>
+  __device__ static int chunk_off(int gp, int pp, int cu) {
+    const int c = 4 * pp + cu;
+    return gp * PG + 4 * (SWZ ? c ^ (gp ^ (pp & 2)) : c);
+  }
   __device__ static int col_block(int wave) { return NRB == 4 ? wave : (NRB == 2 ? (wave & 1) : 0); }
   __device__ static int row_block(int wave, int j) { return NRB == 4 ? j : (NRB == 2 ? (wave >> 1) : 0); }
 };
@@ -820,11 +830,13 @@ struct DW2Own {
 // dW2[j] += G_c a1_c^T over the tile.  a1_c of unit u = 16 ibw + pl is
 // rebuilt from h: KIND 0: h, 1: (1 - h^2) sc, 2: h (1 - h^2) sc with sc
 // supplied by the caller (zt_i and -2 zt_i^2 for unit u).
-// stgl = stg + 16 g + pl + (NRB == 2 ? row block (wave >> 1) * BS : 0) and
-// stbl = stb + ibw * BS + 16 g + pl are this lane's read bases (wave w,
-// pair 4ks + g at offset w * WS + ks * PG, row block ob at ob * BS).
+// stgl[ks] = stg + (NRB == 2 ? row block (wave >> 1) * BS : 0) +
+// chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3) and stbl = stb + ibw * BS +
+// 16 g + pl are this lane's read bases for pair 4 ks + g, unit pl (source
+// wave w at + w * WS, row block ob at + ob * BS; stb pair group ks at
+// + ks * PG).
 template <int H, int KIND>
-__device__ inline void dw2_shared(const float* stgl, const float* stbl, float sc, f4* dW2, int wave) {
+__device__ inline void dw2_shared(const float* const* stgl, const float* stbl, float sc, f4* dW2, int wave) {
   using O = DW2Own<H>;
 #pragma unroll
   for (int s = 0; s < O::NSRC; ++s) {
@@ -837,7 +849,7 @@ __device__ inline void dw2_shared(const float* stgl, const float* stbl, float sc
       if (KIND == 2) b = h * fmaf(-h, h, 1.f) * sc;
 #pragma unroll
       for (int j = 0; j < O::NOB; ++j) {
-        const float a = stgl[(O::NOB == 1 ? 0 : j * O::BS) + w * O::WS + ks * O::PG];
+        const float a = stgl[ks][(O::NOB == 1 ? 0 : j * O::BS) + w * O::WS];
         dW2[j] = mfma16(a, b, dW2[j]);
       }
     }
@@ -900,11 +912,17 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   const int ucol = 16 * O::col_block(wave) + pl;   // this lane's dW2 column unit
   const float wl_red = ss[SM::WL + 16 * (pl >> 2) + 4 * g + (pl & 3)];   // WL of the unit a reduction leaves here
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
-  const int st_lane = wave * O::WS + (pl >> 2) * O::PG + (pl & 3) * 16 + 4 * g;
-  float* const stbw = stb + st_lane;                                 // store bases
-  float* const stgw = stg + st_lane;
+  // the G buffer (five stores per tile) is swizzled at h = 64 where the four
+  // extra read bases fit the register budget (every first-derivative axis
+  // also carries its second derivative)
+  constexpr bool SWZ = H == 64 && D1M == D2M;
+  float* const stbw = stb + wave * O::WS + O::template chunk_off<false>(pl >> 2, pl & 3, g);   // store bases
+  float* const stgw = stg + wave * O::WS + O::template chunk_off<SWZ>(pl >> 2, pl & 3, g);
   const float* const stbl = stb + O::col_block(wave) * O::BS + 16 * g + pl;   // dW2 read bases
-  const float* const stgl = stg + (H == 32 ? (wave >> 1) * O::BS : 0) + 16 * g + pl;
+  const float* stgl[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks)
+    stgl[ks] = stg + (H == 32 ? (wave >> 1) * O::BS : 0) + O::template chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3);
   float* mysmall = ssmall + wave * NS * H;
   f4 dW2[O::NOB];
 #pragma unroll
