@@ -77,7 +77,7 @@ class FusedEngine:
     """Owns the packed device state and runs train steps for one problem."""
 
     def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
-                 graph=True, autotune=True):
+                 graph=True, autotune=True, fused_loss=True):
         from fbpinns_amd.decompositions import RectangularDecompositionND
         from fbpinns_amd.networks import FCN
         if not torch.cuda.is_available():
@@ -90,6 +90,7 @@ class FusedEngine:
         self.device = torch.device(device)
         self.graph_enabled = bool(graph)
         self.autotune_enabled = bool(autotune)
+        self.fused_loss = bool(fused_loss)
         ok = dict(optimiser_kwargs or {})
         self.lr = float(ok.pop("learning_rate", 1e-3))
         self.b1 = float(ok.pop("b1", 0.9))
@@ -246,7 +247,8 @@ class FusedEngine:
         self.active = active_
         self.x = x
         self.takes = takes
-        self.epilogue = Epilogue(self.problem, self.all_params, x, cranges, cvalues, self.required_ujss, self.net)
+        self.epilogue = Epilogue(self.problem, self.all_params, x, cranges, cvalues, self.required_ujss, self.net,
+                                 fused_loss=self.fused_loss)
         self.n_active = n_active
         logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks, "
                     f"fwd {autotune.name(self.launch[0])} ({plan.wl.n_fbins} workgroups), "

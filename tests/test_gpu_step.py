@@ -66,7 +66,7 @@ CASES = {
 }
 
 
-def _build(name, seed=0, graph=False):
+def _build(name, seed=0, graph=False, fused_loss=True):
     prob, kw, lo, hi, nsub, ov, ls, ns, sampler, ofac, has_mu = CASES[name]
     if ov is None:   # cfg 1: 15 centres, width 0.15
         from fbpinns_amd.decompositions import RectangularDecompositionND
@@ -89,7 +89,7 @@ def _build(name, seed=0, graph=False):
     constraints = [list(c[:-1]) for c in cg]
     req = [c[-1] for c in cg]
     eng = FusedEngine(prob, dec_cls, FCN, ap, optimiser_kwargs=dict(learning_rate=1e-3), device="cuda:0",
-                      graph=graph)
+                      graph=graph, fused_loss=fused_loss)
     eng.set_constraints(constraints, req)
     eng.set_active(np.ones(eng.m, dtype=int))
     return eng, ap, constraints, ofac, has_mu
@@ -271,3 +271,21 @@ def test_plain_form_matches_fused(name, chunks):
     W = pl.layers[0][0]
     Wf = eng.all_params["trainable"]["network"]["subdomain"]["layers"][0][0]
     assert (W - Wf).abs().max().item() <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["ho_inverse_mu", "wave_small", "burgers_small"])
+def test_fused_residual_loss_steps_match_loss_fn(name):
+    """Graph-replayed steps with the fused residual-loss kernels follow the
+    same trajectory as steps that run the problem's loss_fn."""
+    ea, *_ = _build(name, graph=True)
+    eb, *_ = _build(name, graph=True, fused_loss=False)
+    assert ea.epilogue.fused is not None and eb.epilogue.fused is None
+    for k in range(3):
+        la, lb = ea.step().item(), eb.step().item()
+        e = abs(la - lb) / abs(lb)
+        print(f"[fused loss] {name} step {k}: fused {la:.8e} loss_fn {lb:.8e} rel {e:.2e}", flush=True)
+        assert e <= TOL
+    W = ea.all_params["trainable"]["network"]["subdomain"]["layers"][0][0]
+    Wb = eb.all_params["trainable"]["network"]["subdomain"]["layers"][0][0]
+    assert (W - Wb).abs().max().item() <= 1e-5
+    assert torch.allclose(ea.pflat, eb.pflat, rtol=1e-5, atol=1e-7)
