@@ -369,23 +369,28 @@ __global__ void __launch_bounds__(256) adam_flat_kernel(float* __restrict__ p, f
 }
 
 // Deterministic mean: stage 1 sums a fixed grid of strided ranges into
-// scratch[kSumBlocks] with an LDS tree, stage 2 reduces those in one block.
+// scratch[kSumBlocks] with block sums, stage 2 reduces those in one block.
 // No atomics and no memset, so it is safe inside a captured graph.
 constexpr int kSumBlocks = 256;
 
+// Sum of s over the block in a fixed order: a butterfly over each wave's 64
+// lanes, then the waves' partials in wave order.  The total is returned in
+// thread 0 only.  sm holds blockDim.x / 64 floats; a caller that reuses sm
+// separates the calls by a barrier.
 __device__ inline float block_tree_sum(float s, float* sm) {
-  sm[threadIdx.x] = s;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
-    __syncthreads();
-  }
-  return sm[0];
+  float t = 0.f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sm[w];
+  return t;
 }
 
 __global__ void __launch_bounds__(256) sum_stage1_kernel(const float* __restrict__ x, int64_t n,
                                                          float* __restrict__ scratch) {
-  __shared__ float sm[256];
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
   float s = 0.f;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     s += x[i];
@@ -395,7 +400,7 @@ __global__ void __launch_bounds__(256) sum_stage1_kernel(const float* __restrict
 
 __global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict__ scratch, int nb, int64_t n,
                                                          float* __restrict__ out) {
-  __shared__ float sm[256];
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const float s = (int)threadIdx.x < nb ? scratch[threadIdx.x] : 0.f;
   const float t = block_tree_sum(s, sm);
   if (threadIdx.x == 0) out[0] = t / (float)n;
@@ -403,71 +408,81 @@ __global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict
 
 // ------------------------------------------------------------------ fused residual loss
 
-// Column and term tables of one residual; d*: per column f, the terms in
-// which it appears, as (term, other factor column or -1) in
-// dterm/dother[dstart[f] .. dstart[f+1]-1] (a term col_f * col_f appears twice).
+// Column and term tables of one residual (factor index -1: the factor 1).
 struct ResidualArgs {
   int n_cols, n_terms;
   const float* col[FBPX_MAX_REQ];
   float* gcol[FBPX_MAX_REQ];
   int fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
-  int dstart[FBPX_MAX_REQ + 1];
-  int dterm[2 * FBPX_MAX_TERMS], dother[2 * FBPX_MAX_TERMS];
 };
 
 // Points per thread and loop trip: independent loads in flight per thread.
 constexpr int kResUnroll = 4;
 
-// Term t's factor columns (nullptr: factor 1), uniform across the grid.
+// The factor 1 of a term without a second (or any) column is read from here
+// at index 0, so that every factor is an unconditional load and all of a
+// point's loads are in flight at once (a load behind a branch on the term
+// count would be waited for before the next one is issued).
+__device__ float kResOne[1] = {1.f};
+
+// Term t's factor columns and index masks (all-ones for a column, 0 for the
+// factor 1, which reads kResOne[0]), uniform across the grid.
+template <int NT>
 struct TermCols {
-  const float* pa[FBPX_MAX_TERMS];
-  const float* pb[FBPX_MAX_TERMS];
+  const float* pa[NT];
+  const float* pb[NT];
+  int64_t ma[NT], mb[NT];
 };
 
-__device__ inline TermCols term_cols(const ResidualArgs& a) {
-  TermCols c;
+static_assert(FBPX_MAX_TERMS == 8, "the residual launches below instantiate 1 .. 8 terms");
+
+template <int NT>
+__device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
+  TermCols<NT> c;
 #pragma unroll
-  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
-    c.pa[t] = t < a.n_terms && a.fa[t] >= 0 ? a.col[a.fa[t]] : nullptr;
-    c.pb[t] = t < a.n_terms && a.fb[t] >= 0 ? a.col[a.fb[t]] : nullptr;
+  for (int t = 0; t < NT; ++t) {
+    const bool ha = a.fa[t] >= 0, hb = a.fb[t] >= 0;
+    c.pa[t] = ha ? a.col[a.fa[t]] : kResOne;
+    c.pb[t] = hb ? a.col[a.fb[t]] : kResOne;
+    c.ma[t] = ha ? ~int64_t(0) : 0;
+    c.mb[t] = hb ? ~int64_t(0) : 0;
   }
   return c;
 }
 
-__device__ inline float term_at(const TermCols& c, int t, int64_t k) {
-  const float x = c.pa[t] ? c.pa[t][k] : 1.f;
-  return c.pb[t] ? x * c.pb[t][k] : x;
-}
-
-// r[k] = sum_t coef[t] term_t(k); scratch[block] = sum of r^2 over the
-// block's grid-stride points.
+// r[k] = sum_t coef[t] term_t(k) for NT = n_terms; scratch[block] = sum of
+// r^2 over the block's grid-stride points.
+template <int NT>
 __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
                                                        float* __restrict__ r, float* __restrict__ scratch) {
-  __shared__ float sm[256];
-  const TermCols c = term_cols(a);
-  float cf[FBPX_MAX_TERMS];
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  const TermCols<NT> c = term_cols<NT>(a);
+  float cf[NT];
 #pragma unroll
-  for (int t = 0; t < FBPX_MAX_TERMS; ++t) cf[t] = t < a.n_terms ? coef[t] : 0.f;
+  for (int t = 0; t < NT; ++t) cf[t] = coef[t];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   float s = 0.f;
   for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
-    float v[kResUnroll];
+    float xa[kResUnroll][NT], xb[kResUnroll][NT];
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
-      v[u] = 0.f;
-      if (k < n) {
+      const int64_t kk = k < n ? k : 0;   // past the end: read point 0, result dropped
 #pragma unroll
-        for (int t = 0; t < FBPX_MAX_TERMS; ++t)
-          if (t < a.n_terms) v[u] = fmaf(cf[t], term_at(c, t, k), v[u]);
+      for (int t = 0; t < NT; ++t) {
+        xa[u][t] = c.pa[t][kk & c.ma[t]];
+        xb[u][t] = c.pb[t][kk & c.mb[t]];
       }
     }
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
+      float v = 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) v = fmaf(cf[t], xa[u][t] * xb[u][t], v);
       if (k < n) {
-        r[k] = v[u];
-        s = fmaf(v[u], v[u], s);
+        r[k] = v;
+        s = fmaf(v, v, s);
       }
     }
   }
@@ -476,58 +491,66 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
 }
 
 // gcol[f][k] = s r[k] d r / d col_f, s = g_loss[0] * w2n; scratch[block *
-// n_terms + t] = sum over the block's points of s r term_t.
+// NT + t] = sum over the block's points of s r term_t.
+template <int NT>
 __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, ResidualArgs a,
                                                                 const float* __restrict__ coef,
                                                                 const float* __restrict__ r, float w2n,
                                                                 const float* __restrict__ g_loss,
                                                                 float* __restrict__ scratch) {
-  __shared__ float sm[256];
-  const TermCols c = term_cols(a);
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  const TermCols<NT> c = term_cols<NT>(a);
   const float sc = g_loss[0] * w2n;
-  float cf[FBPX_MAX_TERMS], acc[FBPX_MAX_TERMS];
+  float cf[NT], acc[NT];
 #pragma unroll
-  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
-    cf[t] = t < a.n_terms ? coef[t] : 0.f;
+  for (int t = 0; t < NT; ++t) {
+    cf[t] = coef[t];
     acc[t] = 0.f;
   }
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
-    float sr[kResUnroll];
+    float xa[kResUnroll][NT], xb[kResUnroll][NT], sr[kResUnroll];
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
-      sr[u] = k < n ? sc * r[k] : 0.f;
+      const int64_t kk = k < n ? k : 0;
+      sr[u] = r[kk];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        xa[u][t] = c.pa[t][kk & c.ma[t]];
+        xb[u][t] = c.pb[t][kk & c.mb[t]];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      sr[u] = k < n ? sc * sr[u] : 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = fmaf(sr[u], xa[u][t] * xb[u][t], acc[t]);
+    }
+    // d r / d col_f = sum over the terms that read col_f of coef times the
+    // other factor (a column read by both factors of a term counts twice);
+    // no loads in this loop, so a runtime column count costs nothing
     for (int f = 0; f < a.n_cols; ++f) {
       float* gc = a.gcol[f];
       if (gc == nullptr) continue;
 #pragma unroll
       for (int u = 0; u < kResUnroll; ++u) {
         const int64_t k = k0 + u * stride;
-        if (k >= n) continue;
         float g = 0.f;
-        for (int e = a.dstart[f]; e < a.dstart[f + 1]; ++e) {
-          const float other = a.dother[e] >= 0 ? a.col[a.dother[e]][k] : 1.f;
-          g = fmaf(cf[a.dterm[e]], other, g);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          if (a.fa[t] == f) g = fmaf(cf[t], xb[u][t], g);
+          if (a.fb[t] == f) g = fmaf(cf[t], xa[u][t], g);
         }
-        gc[k] = sr[u] * g;
+        if (k < n) gc[k] = sr[u] * g;
       }
-    }
-#pragma unroll
-    for (int u = 0; u < kResUnroll; ++u) {
-      const int64_t k = k0 + u * stride;
-      if (k >= n) continue;
-#pragma unroll
-      for (int t = 0; t < FBPX_MAX_TERMS; ++t)
-        if (t < a.n_terms) acc[t] = fmaf(sr[u], term_at(c, t, k), acc[t]);
     }
   }
 #pragma unroll
-  for (int t = 0; t < FBPX_MAX_TERMS; ++t) {
-    if (t >= a.n_terms) break;
+  for (int t = 0; t < NT; ++t) {
     const float tot = block_tree_sum(acc[t], sm);
-    if (threadIdx.x == 0) scratch[blockIdx.x * a.n_terms + t] = tot;
+    if (threadIdx.x == 0) scratch[blockIdx.x * NT + t] = tot;
     __syncthreads();
   }
 }
@@ -536,7 +559,7 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
 // block; thread i sums b = i, i + 256, ... in order, then a tree).
 __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restrict__ scratch, int nb, int ncol,
                                                           float scale, float* __restrict__ out) {
-  __shared__ float sm[256];
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
   for (int j = 0; j < ncol; ++j) {
     float v = 0.f;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) v += scratch[b * ncol + j];
@@ -551,7 +574,7 @@ __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restric
 __global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restrict__ seg_start,
                                                           const int32_t* __restrict__ perm,
                                                           const float* __restrict__ x, float* __restrict__ out) {
-  __shared__ float sm[256];
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const int s = blockIdx.x;
   const int j0 = seg_start[s], j1 = seg_start[s + 1];
   float v = 0.f;
@@ -838,21 +861,6 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int
     a->fa[t] = fa[t] < 0 ? -1 : fa[t];
     a->fb[t] = fb[t] < 0 ? -1 : fb[t];
   }
-  int e = 0;
-  for (int f = 0; f < n_cols; ++f) {
-    a->dstart[f] = e;
-    for (int t = 0; t < n_terms; ++t) {
-      if (a->fa[t] == f) {
-        a->dterm[e] = t;
-        a->dother[e++] = a->fb[t];
-      }
-      if (a->fb[t] == f) {
-        a->dterm[e] = t;
-        a->dother[e++] = a->fa[t];
-      }
-    }
-  }
-  a->dstart[n_cols] = e;
   return FBPX_OK;
 }
 
@@ -868,7 +876,15 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
   }
   hipStream_t s = (hipStream_t)stream;
   const int nb = residual_blocks(n);
-  hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch);
+#define FBPX_RES_CASE(NT)                                                                          \
+  case NT:                                                                                         \
+    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch);    \
+    break;
+  switch (n_terms) {
+    FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
+    FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
+  }
+#undef FBPX_RES_CASE
   hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, weight / (float)n, loss);
   return hip_check(hipGetLastError(), "residual_loss");
 }
@@ -886,8 +902,17 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
   }
   hipStream_t s = (hipStream_t)stream;
   const int nb = residual_blocks(n);
-  hipLaunchKernelGGL(residual_backward_kernel, dim3(nb), dim3(256), 0, s, n, a, coef, r, 2.f * weight / (float)n,
-                     g_loss, scratch);
+  const float w2n = 2.f * weight / (float)n;
+#define FBPX_RES_CASE(NT)                                                                                 \
+  case NT:                                                                                                \
+    hipLaunchKernelGGL(residual_backward_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, w2n, g_loss, \
+                       scratch);                                                                          \
+    break;
+  switch (n_terms) {
+    FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
+    FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
+  }
+#undef FBPX_RES_CASE
   if (gcoef) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
   return hip_check(hipGetLastError(), "residual_loss_backward");
 }
