@@ -55,9 +55,9 @@ SIGNATURES = {
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                _c_p, _c_p, _c_p]),
     "fbpx_reduce_rows": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p]),
-    "fbpx_adam_rows": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_i32, _c_p,
+    "fbpx_adam_rows": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p,
                                 _c_d, _c_d, _c_d, _c_d]),
-    "fbpx_adam_flat": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_d, _c_d, _c_d, _c_d]),
+    "fbpx_adam_flat": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_d, _c_d, _c_d, _c_d]),
     "fbpx_count_increment": (_c_i32, [_c_p, _c_p]),
     "fbpx_mean": (_c_i32, [_c_p, _c_p, ctypes.c_int64, _c_p, _c_p]),
     "fbpx_affine_jets": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_i32, _c_i32, _c_p, _c_p, _c_p, _c_i32,

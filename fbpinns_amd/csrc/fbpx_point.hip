@@ -328,11 +328,25 @@ __global__ void __launch_bounds__(256) reduce_rows_kernel(int P, const float* __
   grad[(size_t)rows[row] * P + e] = g;
 }
 
+// Each workgroup's thread 0 reads the step count once, at the start.  With
+// count_done != NULL that thread, after its read, counts the workgroup on
+// count_done; the last workgroup to arrive adds 1 to the count (every other
+// workgroup has read it by then) and resets count_done to 0 for the next
+// launch.  Nothing else is ordered against the count, so no fence is needed.
+constexpr int kAdamBlocks = 2048;
+
+__device__ inline void adam_count_step(int32_t* count, int32_t* count_done) {
+  if (count_done != nullptr && threadIdx.x == 0 && atomicAdd(count_done, 1) == (int)gridDim.x - 1) {
+    atomicAdd(count, 1);
+    atomicExch(count_done, 0);
+  }
+}
+
 __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ theta, float* __restrict__ mu,
                                                         float* __restrict__ nu, int P,
                                                         const float* __restrict__ grad,
                                                         const int32_t* __restrict__ rows, int n_rows,
-                                                        const int32_t* __restrict__ count, AdamConst k) {
+                                                        int32_t* count, int32_t* count_done, AdamConst k) {
   // bias corrections once per workgroup
   __shared__ float sbc[2];
   if (threadIdx.x == 0) {
@@ -341,31 +355,40 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
     sbc[1] = 1.f - powf(k.b2, cf);
   }
   __syncthreads();
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n_rows * P) return;
-  const int row = (int)(idx / P), e = (int)(idx % P);
   const float bc1 = sbc[0], bc2 = sbc[1];
-  const size_t i = (size_t)rows[row] * P + e;
-  float p = theta[i], m = mu[i], v = nu[i];
-  adam_elem(p, m, v, grad[i], bc1, bc2, k);
-  theta[i] = p;
-  mu[i] = m;
-  nu[i] = v;
+  const int64_t total = (int64_t)n_rows * P;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(idx / P), e = (int)(idx % P);
+    const size_t i = (size_t)rows[row] * P + e;
+    float p = theta[i], m = mu[i], v = nu[i];
+    adam_elem(p, m, v, grad[i], bc1, bc2, k);
+    theta[i] = p;
+    mu[i] = m;
+    nu[i] = v;
+  }
+  adam_count_step(count, count_done);
 }
 
 __global__ void __launch_bounds__(256) adam_flat_kernel(float* __restrict__ p, float* __restrict__ mu,
                                                         float* __restrict__ nu, const float* __restrict__ g, int n,
-                                                        const int32_t* __restrict__ count, AdamConst k) {
+                                                        int32_t* count, int32_t* count_done, AdamConst k) {
+  __shared__ float sbc[2];
+  if (threadIdx.x == 0) {
+    const float cf = (float)(count[0] + 1);
+    sbc[0] = 1.f - powf(k.b1, cf);
+    sbc[1] = 1.f - powf(k.b2, cf);
+  }
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float cf = (float)(count[0] + 1);
-  const float bc1 = 1.f - powf(k.b1, cf);
-  const float bc2 = 1.f - powf(k.b2, cf);
-  float pv = p[i], m = mu[i], v = nu[i];
-  adam_elem(pv, m, v, g[i], bc1, bc2, k);
-  p[i] = pv;
-  mu[i] = m;
-  nu[i] = v;
+  if (i < n) {
+    float pv = p[i], m = mu[i], v = nu[i];
+    adam_elem(pv, m, v, g[i], sbc[0], sbc[1], k);
+    p[i] = pv;
+    mu[i] = m;
+    nu[i] = v;
+  }
+  adam_count_step(count, count_done);
 }
 
 // Deterministic mean: stage 1 sums a fixed grid of strided ranges into
@@ -713,36 +736,47 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial, const int32_
 }
 
 int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
-                   const int32_t* rows, int32_t n_rows, const int32_t* count, double lr, double b1, double b2,
-                   double eps) {
+                   const int32_t* rows, int32_t n_rows, int32_t* count, int32_t* count_done, double lr, double b1,
+                   double b2, double eps) {
   if (P <= 0 || n_rows < 0) {
     set_error("adam_rows: bad sizes P=%d n_rows=%d", P, n_rows);
     return FBPX_EINVAL;
   }
-  if (n_rows == 0) return FBPX_OK;
-  if (!theta || !mu || !nu || !grad || !rows || !count) {
+  if (!count) {
+    set_error("adam_rows: null count");
+    return FBPX_EINVAL;
+  }
+  if (n_rows == 0) return count_done ? fbpx_count_increment(stream, count) : FBPX_OK;
+  if (!theta || !mu || !nu || !grad || !rows) {
     set_error("adam_rows: null pointer");
     return FBPX_EINVAL;
   }
-  const int64_t total = (int64_t)n_rows * P;
-  hipLaunchKernelGGL(adam_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                     theta, mu, nu, P, grad, rows, n_rows, count, adam_const(lr, b1, b2, eps));
+  // grid-stride over at most kAdamBlocks workgroups: few arrivals on the
+  // count_done counter, and enough workgroups to fill the GPU
+  const int64_t total = (int64_t)n_rows * P, need = (total + 255) / 256;
+  const unsigned nb = (unsigned)(need < kAdamBlocks ? need : kAdamBlocks);
+  hipLaunchKernelGGL(adam_rows_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                     theta, mu, nu, P, grad, rows, n_rows, count, count_done, adam_const(lr, b1, b2, eps));
   return hip_check(hipGetLastError(), "adam_rows");
 }
 
-int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g, int32_t n, const int32_t* count,
-                   double lr, double b1, double b2, double eps) {
+int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g, int32_t n, int32_t* count,
+                   int32_t* count_done, double lr, double b1, double b2, double eps) {
   if (n < 0) {
     set_error("adam_flat: n=%d", n);
     return FBPX_EINVAL;
   }
-  if (n == 0) return FBPX_OK;
-  if (!p || !mu || !nu || !g || !count) {
+  if (!count) {
+    set_error("adam_flat: null count");
+    return FBPX_EINVAL;
+  }
+  if (n == 0) return count_done ? fbpx_count_increment(stream, count) : FBPX_OK;
+  if (!p || !mu || !nu || !g) {
     set_error("adam_flat: null pointer");
     return FBPX_EINVAL;
   }
   hipLaunchKernelGGL(adam_flat_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, mu, nu, g, n,
-                     count, adam_const(lr, b1, b2, eps));
+                     count, count_done, adam_const(lr, b1, b2, eps));
   return hip_check(hipGetLastError(), "adam_flat");
 }
 

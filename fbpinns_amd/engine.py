@@ -10,11 +10,12 @@ State lives on one GPU in packed form:
                 problem), with their Adam moments.
   dom [m][12]   xmin, xmax, unnorm mean/sd and window flag per subdomain.
 
-One step = fbpx_forward -> fbpx_pou_reduce -> loss epilogue (torch on the
-per-point jets) -> autograd (fbpx_pou_backward -> fbpx_backward ->
-fbpx_reduce_rows, plus torch for the epilogue) -> fbpx_adam_rows over the
-trained rows (+ fbpx_adam_flat) -> fbpx_count_increment.  With graph=True
-the whole step is captured once per work list into a hipGraph and replayed.
+One step = fbpx_forward -> fbpx_pou_reduce -> loss epilogue (fbpx_affine_jets
+and fbpx_residual_loss, or torch on the per-point jets) -> autograd
+(the epilogue's transposes -> fbpx_pou_backward -> fbpx_backward ->
+fbpx_reduce_rows) -> fbpx_adam_rows over the trained rows (+ fbpx_adam_flat),
+whose last launch also advances the shared step count.  With graph=True the
+whole step is captured once per work list into a hipGraph and replayed.
 """
 
 import numpy as np
@@ -126,6 +127,8 @@ class FusedEngine:
         self.mu = torch.zeros_like(self.theta)
         self.nu = torch.zeros_like(self.theta)
         self.count = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.count_done = torch.zeros(1, dtype=torch.int32, device=self.device)   # 0 between launches
+        self.loss_seed = torch.ones((), dtype=torch.float32, device=self.device)
 
         # problem trainable leaves -> one flat vector with views
         prob = ap["trainable"].get("problem", {})
@@ -243,6 +246,7 @@ class FusedEngine:
         del scratch
         self.plan = ops.Plan(self.net, self.dom, x, plan.wl, wsum, plan.chunk_train, plan.rows,
                              plan.row_chunk_start, self.m)
+        self.plan.keep_gradient_buffer(self.net.P)
         self.workplan = plan
         self.active = active_
         self.x = x
@@ -276,7 +280,7 @@ class FusedEngine:
         g_p = None
         if pleaf is not None:
             g_p = grads[1] if grads[1] is not None else torch.zeros_like(self.pflat)
-        return loss.detach(), grads[0], g_p
+        return loss.detach(), grads[0].clone(), g_p
 
     def _step_body(self):
         plan = self.plan
@@ -284,19 +288,21 @@ class FusedEngine:
         ujet = FBPINNJets.apply(th, plan)
         params, pleaf = self._params_for_step()
         loss = self.epilogue.loss(ujet, params)
+        # the preallocated seed d loss / d loss = 1 keeps a fill kernel out of the step
         if pleaf is not None:
-            g_theta, g_p = torch.autograd.grad(loss, [th, pleaf], allow_unused=True)
+            g_theta, g_p = torch.autograd.grad(loss, [th, pleaf], grad_outputs=(self.loss_seed,),
+                                               allow_unused=True)
         else:
-            (g_theta,) = torch.autograd.grad(loss, [th])
+            (g_theta,) = torch.autograd.grad(loss, [th], grad_outputs=(self.loss_seed,))
             g_p = None
-        ops.adam_rows(self.theta, self.mu, self.nu, g_theta, plan.rows, self.count, self.lr, self.b1, self.b2,
-                      self.eps)
+        # the step's last Adam launch also advances the shared count
+        ops.adam_rows(self.theta, self.mu, self.nu, g_theta, plan.rows, self.count, self.count_done, not self.Q,
+                      self.lr, self.b1, self.b2, self.eps)
         if self.Q:
             if g_p is None:
                 g_p = torch.zeros_like(self.pflat)
-            ops.adam_flat(self.pflat, self.pmu, self.pnu, g_p.contiguous(), self.count, self.lr, self.b1, self.b2,
-                          self.eps)
-        ops.count_increment(self.count)
+            ops.adam_flat(self.pflat, self.pmu, self.pnu, g_p.contiguous(), self.count, self.count_done, True,
+                          self.lr, self.b1, self.b2, self.eps)
         return loss.detach()
 
     def _state(self):

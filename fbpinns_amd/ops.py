@@ -211,22 +211,27 @@ def reduce_rows(partial: torch.Tensor, rows: torch.Tensor, row_chunk_start: torc
                                  _p(grad, torch.float32, "grad")), "fbpx_reduce_rows")
 
 
-@torch.library.custom_op("fbpx::adam_rows", mutates_args=("theta", "mu", "nu"))
+@torch.library.custom_op("fbpx::adam_rows", mutates_args=("theta", "mu", "nu", "count", "count_done"))
 def adam_rows(theta: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, grad: torch.Tensor, rows: torch.Tensor,
-              count: torch.Tensor, lr: float, b1: float, b2: float, eps: float) -> None:
+              count: torch.Tensor, count_done: torch.Tensor, advance: bool, lr: float, b1: float, b2: float,
+              eps: float) -> None:
+    """advance: this launch also adds 1 to count (count_done: int32 0)."""
     P = theta.shape[1]
     check(lib().fbpx_adam_rows(_stream(), _p(theta, torch.float32, "theta"), _p(mu, torch.float32, "mu"),
                                _p(nu, torch.float32, "nu"), P, _p(grad, torch.float32, "grad"),
                                _p(rows, torch.int32, "rows"), rows.shape[0], _p(count, torch.int32, "count"),
-                               lr, b1, b2, eps), "fbpx_adam_rows")
+                               _p(count_done, torch.int32, "count_done") if advance else None, lr, b1, b2, eps),
+          "fbpx_adam_rows")
 
 
-@torch.library.custom_op("fbpx::adam_flat", mutates_args=("p", "mu", "nu"))
+@torch.library.custom_op("fbpx::adam_flat", mutates_args=("p", "mu", "nu", "count", "count_done"))
 def adam_flat(p: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, g: torch.Tensor, count: torch.Tensor,
-              lr: float, b1: float, b2: float, eps: float) -> None:
+              count_done: torch.Tensor, advance: bool, lr: float, b1: float, b2: float, eps: float) -> None:
     check(lib().fbpx_adam_flat(_stream(), _p(p, torch.float32, "p"), _p(mu, torch.float32, "mu"),
                                _p(nu, torch.float32, "nu"), _p(g, torch.float32, "g"), p.numel(),
-                               _p(count, torch.int32, "count"), lr, b1, b2, eps), "fbpx_adam_flat")
+                               _p(count, torch.int32, "count"),
+                               _p(count_done, torch.int32, "count_done") if advance else None, lr, b1, b2, eps),
+          "fbpx_adam_flat")
 
 
 @torch.library.custom_op("fbpx::count_increment", mutates_args=("count",))
@@ -498,6 +503,13 @@ class Plan:
         self.pairbuf = torch.empty((max(wl.n_pairs, 1), net.J), dtype=torch.float32, device=x.device)
         self.partial = torch.empty((max(wl.n_chunks, 1), net.P), dtype=torch.float32, device=x.device)
         self.gseg = torch.empty((max(wl.n_segs, 1), net.J), dtype=torch.float32, device=x.device)
+        # set by the engine for its step plan: the backward returns this one
+        # [m][P] buffer every time (untrained rows stay 0, so no zero fill per
+        # step); callers that keep a gradient across steps copy it
+        self.g_theta = None
+
+    def keep_gradient_buffer(self, P):
+        self.g_theta = torch.zeros((self.m, P), dtype=torch.float32, device=self.x.device)
 
 
 class FBPINNJets(torch.autograd.Function):
@@ -525,6 +537,8 @@ class FBPINNJets(torch.autograd.Function):
         pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, g_ujet, plan.gseg)
         backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                  plan.gseg, plan.partial)
-        g_theta = torch.zeros_like(theta)
+        g_theta = plan.g_theta
+        if g_theta is None or g_theta.shape != theta.shape:
+            g_theta = torch.zeros_like(theta)
         reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
         return g_theta, None

@@ -176,15 +176,18 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial,
  * Rows not listed (fixed or discarded subdomains) are untouched.
  * theta, mu, nu, grad: [m][P].  The hyper-parameters are doubles so that
  * f32(1 - b1) and f32(-lr) are formed as the reference forms them (Python
- * floats meeting FP32 arrays). */
+ * floats meeting FP32 arrays).  count_done: NULL, or a DEVICE int32 that is 0
+ * before the call (the call leaves it 0): then the same launch also does
+ * *count += 1 once every workgroup has read the count (the step's last
+ * Adam launch carries it, in place of fbpx_count_increment). */
 int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu,
                    int32_t P, const float* grad, const int32_t* rows, int32_t n_rows,
-                   const int32_t* count, double lr, double b1, double b2, double eps);
+                   int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps);
 
 /* The same update on a flat vector p[n] with gradient g[n] (the non-subdomain
  * trainable leaves, e.g. a learnable PDE coefficient, problems.py:232-234). */
 int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g,
-                   int32_t n, const int32_t* count, double lr, double b1, double b2,
+                   int32_t n, int32_t* count, int32_t* count_done, double lr, double b1, double b2,
                    double eps);
 
 /* *count += 1 (optax's shared int32 step count, trainers.py:531,640). */
