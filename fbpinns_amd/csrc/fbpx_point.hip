@@ -448,6 +448,33 @@ constexpr int kResUnroll = 4;
 // count would be waited for before the next one is issued).
 __device__ float kResOne[1] = {1.f};
 
+// out[j] = scale * sum over the grid's workgroups b of scratch[b * ncol + j],
+// computed by the last workgroup to finish (counted on done, which it
+// re-zeroes for the next launch) in the order of sum_columns_kernel, so one
+// launch replaces the partial-sum launch and its reduction.  Every thread of
+// every workgroup calls it after thread 0 has written the workgroup's
+// partials.
+__device__ inline void last_block_sum(const float* scratch, int ncol, float scale, float* out, int32_t* done,
+                                      float* sm) {
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();   // this workgroup's partials are visible before it is counted
+    last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();     // every other workgroup's partials are visible here
+  for (int j = 0; j < ncol; ++j) {
+    float v = 0.f;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v += scratch[b * ncol + j];
+    const float t = block_tree_sum(v, sm);
+    if (threadIdx.x == 0) out[j] = scale * t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicExch(done, 0);
+}
+
 // Term t's factor columns and index masks (all-ones for a column, 0 for the
 // factor 1, which reads kResOne[0]), uniform across the grid.
 template <int NT>
@@ -477,7 +504,8 @@ __device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
 // r^2 over the block's grid-stride points.
 template <int NT>
 __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
-                                                       float* __restrict__ r, float* __restrict__ scratch) {
+                                                       float* __restrict__ r, float* scratch, float scale,
+                                                       float* __restrict__ loss, int32_t* done) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const TermCols<NT> c = term_cols<NT>(a);
   float cf[NT];
@@ -511,6 +539,7 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
   }
   const float tot = block_tree_sum(s, sm);
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
+  if (done) last_block_sum(scratch, 1, scale, loss, done, sm);
 }
 
 // gcol[f][k] = s r[k] d r / d col_f, s = g_loss[0] * w2n; scratch[block *
@@ -519,8 +548,8 @@ template <int NT>
 __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, ResidualArgs a,
                                                                 const float* __restrict__ coef,
                                                                 const float* __restrict__ r, float w2n,
-                                                                const float* __restrict__ g_loss,
-                                                                float* __restrict__ scratch) {
+                                                                const float* __restrict__ g_loss, float* scratch,
+                                                                float* __restrict__ gcoef, int32_t* done) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const TermCols<NT> c = term_cols<NT>(a);
   const float sc = g_loss[0] * w2n;
@@ -576,6 +605,7 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
     if (threadIdx.x == 0) scratch[blockIdx.x * NT + t] = tot;
     __syncthreads();
   }
+  if (done) last_block_sum(scratch, NT, 1.f, gcoef, done, sm);
 }
 
 // out[j] = scale * sum over b < nb of scratch[b * ncol + j], j < ncol (one
@@ -900,7 +930,7 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int
 
 int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
                        const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                       float* scratch, float* loss) {
+                       float* scratch, float* loss, int32_t* done) {
   ResidualArgs a;
   int rc = residual_args(n, n_cols, col, n_terms, fa, fb, nullptr, &a);
   if (rc) return rc;
@@ -912,21 +942,22 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
   const int nb = residual_blocks(n);
 #define FBPX_RES_CASE(NT)                                                                          \
   case NT:                                                                                         \
-    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch);    \
+    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, done); \
     break;
+  const float scale = weight / (float)n;
   switch (n_terms) {
     FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
   }
 #undef FBPX_RES_CASE
-  hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, weight / (float)n, loss);
+  if (!done) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss);
   return hip_check(hipGetLastError(), "residual_loss");
 }
 
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
                                 int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
                                 const float* r, float weight, const float* g_loss, float* const* gcol,
-                                float* scratch, float* gcoef) {
+                                float* scratch, float* gcoef, int32_t* done) {
   ResidualArgs a;
   int rc = residual_args(n, n_cols, col, n_terms, fa, fb, gcol, &a);
   if (rc) return rc;
@@ -940,14 +971,15 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
 #define FBPX_RES_CASE(NT)                                                                                 \
   case NT:                                                                                                \
     hipLaunchKernelGGL(residual_backward_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, w2n, g_loss, \
-                       scratch);                                                                          \
+                       scratch, gcoef, gcoef ? done : nullptr);                                           \
     break;
   switch (n_terms) {
     FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
   }
 #undef FBPX_RES_CASE
-  if (gcoef) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
+  if (gcoef && !done)
+    hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
   return hip_check(hipGetLastError(), "residual_loss_backward");
 }
 

@@ -217,7 +217,8 @@ class Epilogue:
 
     def _fused_plans(self, spec, all_params):
         """Per residual: (constraint, weight, column keys, per-point data,
-        factor columns fa / fb, static coefficients, trainable coefficients)."""
+        factor columns fa / fb, static coefficients, trainable coefficients,
+        the zeroed workgroup counter of its single-launch kernels)."""
         plans = []
         for res in spec:
             ic = res.constraint
@@ -264,7 +265,8 @@ class Epilogue:
             static = torch.tensor([0.0 if callable(c) else float(c) for c, _, _ in res.terms], dtype=torch.float32,
                                   device=self.x.device)
             dyn = [(t, c) for t, (c, _, _) in enumerate(res.terms) if callable(c)]
-            plans.append((ic, float(res.weight), keys, data, fa, fb, static, dyn))
+            done = torch.zeros(1, dtype=torch.int32, device=self.x.device)   # single-launch counter
+            plans.append((ic, float(res.weight), keys, data, fa, fb, static, dyn, done))
         return plans
 
     def _fused_agrees(self, spec, all_params, n_values):
@@ -298,7 +300,7 @@ class Epilogue:
 
     def _fused_loss(self, ujss, all_params):
         loss = None
-        for ic, weight, keys, data, fa, fb, static, dyn in self.fused:
+        for ic, weight, keys, data, fa, fb, static, dyn, done in self.fused:
             cols = []
             for k in keys:
                 if k[0] == "J":
@@ -315,7 +317,7 @@ class Epilogue:
                 for t, c in dyn:
                     parts[t] = torch.as_tensor(c(all_params)).reshape(()).to(torch.float32)
                 coef = torch.stack(parts)
-            term = ResidualLoss.apply(coef, weight, fa, fb, *cols)
+            term = ResidualLoss.apply(coef, weight, fa, fb, done, *cols)
             loss = term if loss is None else loss + term
         return loss
 

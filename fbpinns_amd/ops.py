@@ -333,9 +333,10 @@ def _residual_tables(cols, fa, fb, n, name):
     return ptrs, a, b
 
 
-@torch.library.custom_op("fbpx::residual_loss", mutates_args=("r", "scratch", "loss"))
+@torch.library.custom_op("fbpx::residual_loss", mutates_args=("r", "scratch", "loss", "done"))
 def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor, weight: float,
-                  r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor) -> None:
+                  r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor) -> None:
+    """done: int32 0 (left 0), the single-launch workgroup counter."""
     n = r.numel()
     ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss")
     if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS or loss.numel() != 1:
@@ -343,14 +344,15 @@ def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: 
                         f"loss {loss.numel()}")
     check(lib().fbpx_residual_loss(_stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
                                    float(weight), _p(r, torch.float32, "r"), _p(scratch, torch.float32, "scratch"),
-                                   _p(loss, torch.float32, "loss")), "fbpx_residual_loss")
+                                   _p(loss, torch.float32, "loss"), _p(done, torch.int32, "done")),
+          "fbpx_residual_loss")
 
 
-@torch.library.custom_op("fbpx::residual_loss_backward", mutates_args=("gcols", "scratch", "gcoef"))
+@torch.library.custom_op("fbpx::residual_loss_backward", mutates_args=("gcols", "scratch", "gcoef", "done"))
 def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor,
                            r: torch.Tensor, weight: float, g_loss: torch.Tensor, gidx: List[int],
                            gcols: List[torch.Tensor], scratch: torch.Tensor, gcoef: torch.Tensor,
-                           want_coef: bool) -> None:
+                           want_coef: bool, done: torch.Tensor) -> None:
     n = r.numel()
     ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward")
     if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
@@ -367,43 +369,46 @@ def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int
     check(lib().fbpx_residual_loss_backward(
         _stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"), _p(r, torch.float32, "r"),
         float(weight), _p(g_loss, torch.float32, "g_loss"), gp, _p(scratch, torch.float32, "scratch"),
-        _p(gcoef, torch.float32, "gcoef") if want_coef else None), "fbpx_residual_loss_backward")
+        _p(gcoef, torch.float32, "gcoef") if want_coef else None, _p(done, torch.int32, "done")),
+        "fbpx_residual_loss_backward")
 
 
 class ResidualLoss(torch.autograd.Function):
     """weight / n * sum_k (sum_t coef[t] F(fa[t], k) F(fb[t], k))^2 over n
     points (fbpx_residual_loss): cols are (n, 1) or (n,) float32 columns,
-    F(f, k) = cols[f][k] (1 for f < 0), coef an (n_terms,) float32 tensor.
-    Gradients for the columns and coefficients that require them."""
+    F(f, k) = cols[f][k] (1 for f < 0), coef an (n_terms,) float32 tensor,
+    done an int32 0 that the launches leave 0 (one per residual: residuals
+    that may run concurrently need their own).  Gradients for the columns
+    and coefficients that require them."""
 
     @staticmethod
-    def forward(ctx, coef, weight, fa, fb, *cols):
+    def forward(ctx, coef, weight, fa, fb, done, *cols):
         cs = [c.contiguous() for c in cols]
         dev = cs[0].device
         n = cs[0].numel()
         r = torch.empty(n, dtype=torch.float32, device=dev)
         scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=dev)
         loss = torch.empty((), dtype=torch.float32, device=dev)
-        residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss)
-        ctx.save_for_backward(coef, r, *cs)
+        residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss, done)
+        ctx.save_for_backward(coef, r, done, *cs)
         ctx.weight, ctx.fa, ctx.fb = float(weight), list(fa), list(fb)
         return loss
 
     @staticmethod
     def backward(ctx, g):
-        coef, r, *cs = ctx.saved_tensors
+        coef, r, done, *cs = ctx.saved_tensors
         need = ctx.needs_input_grad
-        gidx = [i for i in range(len(cs)) if need[4 + i]]
+        gidx = [i for i in range(len(cs)) if need[5 + i]]
         gcols = [torch.empty_like(cs[i]) for i in gidx]
         nt = len(ctx.fa)
         gcoef = torch.empty(nt, dtype=torch.float32, device=r.device)
         scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=r.device)
         residual_loss_backward(cs, ctx.fa, ctx.fb, coef, r, ctx.weight, g.contiguous(), gidx, gcols, scratch,
-                               gcoef, bool(need[0]))
+                               gcoef, bool(need[0]), done)
         out = [None] * len(cs)
         for i, gc in zip(gidx, gcols):
             out[i] = gc
-        return (gcoef if need[0] else None, None, None, None, *out)
+        return (gcoef if need[0] else None, None, None, None, None, *out)
 
 
 @torch.library.custom_op("fbpx::segment_sum", mutates_args=("out",))

@@ -241,24 +241,28 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * constraint's jets and per-point data such as sources or targets); fa, fb:
  * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
  * on the device).  Writes r[n] and loss[0]; scratch: FBPX_RESIDUAL_BLOCKS floats.
- * Deterministic (fixed-order reductions, no atomics, no memset).
+ * done: NULL (a second one-workgroup launch sums the per-workgroup partials)
+ * or a DEVICE int32 that is 0 before the call and is left 0: the last
+ * workgroup of the one launch sums them.  Either way the sum has a fixed
+ * order (deterministic, no memset).
  */
 #define FBPX_MAX_TERMS 8
 #define FBPX_RESIDUAL_BLOCKS 1024
 int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
                        const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                       float* scratch, float* loss);
+                       float* scratch, float* loss, int32_t* done);
 
 /* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
  * pointer to d L / d loss):
  *   gcol[f][k] = s * r[k] * d r[k] / d col[f][k]   for every f with gcol[f] != NULL,
  *   gcoef[t]   = s * sum over k of r[k] * (product of term t at k)   (gcoef may be NULL).
  * gcol: HOST array of n_cols device pointers (NULL entries: per-point data,
- * no gradient); scratch: FBPX_RESIDUAL_BLOCKS * n_terms floats. */
+ * no gradient); scratch: FBPX_RESIDUAL_BLOCKS * n_terms floats; done as in
+ * fbpx_residual_loss (for the gcoef sums). */
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
                                 int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
                                 const float* r, float weight, const float* g_loss, float* const* gcol,
-                                float* scratch, float* gcoef);
+                                float* scratch, float* gcoef, int32_t* done);
 
 /*
  * Deterministic segmented sum, the transpose of a per-point gather

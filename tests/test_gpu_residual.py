@@ -138,7 +138,8 @@ def test_residual_term_forms():
     cols[0].requires_grad_(True)
     cols[1].requires_grad_(True)
     coef = coef64.float().to(dev).requires_grad_(True)
-    lf = ResidualLoss.apply(coef, 0.25, fa, fb, *cols)
+    done = torch.zeros(1, dtype=torch.int32, device=dev)
+    lf = ResidualLoss.apply(coef, 0.25, fa, fb, done, *cols)
     ga, gb, gc = torch.autograd.grad(lf, [cols[0], cols[1], coef])
     c64 = [c.clone().requires_grad_(i < 2) for i, c in enumerate(cols64)]
     k64 = coef64.clone().requires_grad_(True)
@@ -149,6 +150,10 @@ def test_residual_term_forms():
     assert abs(lf.item() - l64.item()) / l64.item() < 1e-5
     for u, v in ((ga, ra), (gb, rb), (gc, rc)):
         assert ((u.cpu().double() - v).abs().max() / v.abs().max()).item() < 1e-5
+    assert int(done.item()) == 0   # the single-launch counter is left at 0
+    # a second call reuses the counter and gives the same loss, bit for bit
+    lf2 = ResidualLoss.apply(coef, 0.25, fa, fb, done, *cols)
+    assert lf2.item() == lf.item() and int(done.item()) == 0
 
 
 @pytest.mark.parametrize("n,n_vec", [(1, 1), (1000, 7), (70000, 300)])
@@ -170,3 +175,38 @@ def test_gather_transpose_matches_index_add(n, n_vec):
     print(f"[gather] n={n} n_vec={n_vec} rel {e:.1e}", flush=True)
     assert e < 1e-5
     assert (gv[n_vec - 2:].cpu() == 0).all() if n_vec > 2 else True
+
+
+def test_single_launch_sum_equals_two_launch_sum():
+    """fbpx_residual_loss with a workgroup counter (the last workgroup sums
+    the partials) and without one (a second launch sums them): the same
+    fixed-order sum, bit for bit, over 293 workgroups."""
+    import ctypes
+    from fbpinns_amd import _lib
+    dev = torch.device("cuda")
+    n = 300_000
+    g = torch.Generator().manual_seed(5)
+    cols = [torch.randn(n, generator=g).to(dev) for _ in range(2)]
+    coef = torch.tensor([1.0, 0.5], device=dev)
+    fa = (ctypes.c_int32 * 2)(0, 1)
+    fb = (ctypes.c_int32 * 2)(-1, 0)
+    ptrs = (ctypes.c_void_p * 2)(*[c.data_ptr() for c in cols])
+    out = []
+    for use_done in (False, True):
+        r = torch.empty(n, device=dev)
+        scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, device=dev)
+        loss = torch.empty((), device=dev)
+        done = torch.zeros(1, dtype=torch.int32, device=dev)
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        rc = _lib.lib().fbpx_residual_loss(stream, n, 2, ptrs, 2, fa, fb, ctypes.c_void_p(coef.data_ptr()), 0.7,
+                                           ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(scratch.data_ptr()),
+                                           ctypes.c_void_p(loss.data_ptr()),
+                                           ctypes.c_void_p(done.data_ptr()) if use_done else None)
+        assert rc == 0
+        torch.cuda.synchronize()
+        out.append(loss.item())
+        assert int(done.item()) == 0
+    ref = 0.7 * ((cols[0].double() + 0.5 * cols[1].double() * cols[0].double()) ** 2).mean().item()
+    print(f"[residual] two launches {out[0]!r} one launch {out[1]!r} float64 {ref!r}", flush=True)
+    assert out[0] == out[1]
+    assert abs(out[0] - ref) / ref < 1e-5

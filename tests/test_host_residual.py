@@ -142,7 +142,8 @@ def test_fused_plans_structure(name):
     spec = prob.residuals(ap)
     plans = ep._fused_plans(spec, ap)
     assert len(plans) == len(spec)
-    for (ic, weight, keys, data, fa, fb, static, dyn), res in zip(plans, spec):
+    for (ic, weight, keys, data, fa, fb, static, dyn, done), res in zip(plans, spec):
+        assert done.dtype == torch.int32 and int(done.item()) == 0
         n_c = ep.cranges[ic][1]
         assert ic == res.constraint and weight == res.weight
         assert len(fa) == len(fb) == len(res.terms) == static.numel()
