@@ -48,9 +48,10 @@ SIGNATURES = {
                                   _c_p, _c_p]),
     "fbpx_forward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                               _c_p]),
-    "fbpx_pou_reduce": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p]),
+    "fbpx_pou_reduce": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p,
+                                 _c_p, _c_p]),
     "fbpx_pou_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p,
-                                   _c_p]),
+                                   _c_p, _c_p]),
     "fbpx_occupancy": (_c_i32, [ctypes.POINTER(fbpx_net), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32)]),
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                _c_p, _c_p, _c_p]),

@@ -162,9 +162,49 @@ __device__ inline void jet_div_T(const float* gq, const float* W, float* gU) {
   gU[0] = g0 * iw;
 }
 
+// The affine constraining function on all J jet columns of one point (the
+// Leibniz rule of fbpx_affine_jets, same operation order):
+//   C_0 = A_0 + B_0 q_0,  C_a = A_a + B_a q_0 + B_0 q_a,
+//   C_b = A_b + B_b q_0 + 2 B_a q_a + B_0 q_b   (a = col1(i), b = col2(i)).
+template <class S>
+__device__ inline void leibniz(const float* Aj, const float* Bj, const float* q, float* c) {
+#pragma clang fp contract(off)
+  c[0] = Aj[0] + Bj[0] * q[0];
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (!S::has1(i)) continue;
+    const int a = S::col1(i);
+    c[a] = Aj[a] + Bj[a] * q[0] + Bj[0] * q[a];
+    if (S::has2(i)) {
+      const int b = S::col2(i);
+      c[b] = Aj[b] + Bj[b] * q[0] + 2.f * Bj[a] * q[a] + Bj[0] * q[b];
+    }
+  }
+}
+
+// Its transpose: gq = (d C / d q)^T gc.
+template <class S>
+__device__ inline void leibniz_T(const float* Bj, const float* gc, float* gq) {
+  gq[0] = Bj[0] * gc[0];
+#pragma unroll
+  for (int i = 0; i < S::XD; ++i) {
+    if (!S::has1(i)) continue;
+    const int a = S::col1(i);
+    gq[0] += Bj[a] * gc[a];
+    gq[a] = Bj[0] * gc[a];
+    if (S::has2(i)) {
+      const int b = S::col2(i);
+      gq[0] += Bj[b] * gc[b];
+      gq[a] += 2.f * Bj[a] * gc[b];
+      gq[b] = Bj[0] * gc[b];
+    }
+  }
+}
+
 template <class S>
 __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
-                                                         const float* __restrict__ wsum, float* __restrict__ ujet) {
+                                                         const float* __restrict__ wsum, const float* __restrict__ A,
+                                                         const float* __restrict__ B, float* __restrict__ ujet) {
   constexpr int J = S::J;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wl.n_points) return;
@@ -188,11 +228,25 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
   }
   const float inv = 1.f / (float)wl.npou;
 #pragma unroll
-  for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = (wl.npou == 1) ? acc[c] : acc[c] * inv;
+  for (int c = 0; c < J; ++c) acc[c] = (wl.npou == 1) ? acc[c] : acc[c] * inv;
+  if (A != nullptr) {
+    float Aj[J], Bj[J], out[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) {
+      Aj[c] = A[(size_t)c * wl.n_points + p];
+      Bj[c] = B[(size_t)c * wl.n_points + p];
+    }
+    leibniz<S>(Aj, Bj, acc, out);
+#pragma unroll
+    for (int c = 0; c < J; ++c) acc[c] = out[c];
+  }
+#pragma unroll
+  for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = acc[c];
 }
 
 template <class S>
 __global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
+                                                           const float* __restrict__ B,
                                                            const float* __restrict__ g_ujet,
                                                            float* __restrict__ gseg) {
   constexpr int J = S::J;
@@ -201,10 +255,18 @@ __global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, con
   const float inv = 1.f / (float)wl.npou;
   float gq[J];
 #pragma unroll
-  for (int c = 0; c < J; ++c) {
-    const float v = g_ujet[(size_t)c * wl.n_points + p];
-    gq[c] = (wl.npou == 1) ? v : v * inv;
+  for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p];
+  if (B != nullptr) {
+    float Bj[J], gc[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) {
+      Bj[c] = B[(size_t)c * wl.n_points + p];
+      gc[c] = gq[c];
+    }
+    leibniz_T<S>(Bj, gc, gq);
   }
+#pragma unroll
+  for (int c = 0; c < J; ++c) gq[c] = (wl.npou == 1) ? gq[c] : gq[c] * inv;
   for (int seg = wl.point_seg_start[p]; seg < wl.point_seg_start[p + 1]; ++seg) {
     float W[J], gU[J];
 #pragma unroll
@@ -719,22 +781,27 @@ int fbpx_pou_prepare(void* stream, const fbpx_net* net, const float* dom, const 
 }
 
 int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* pairbuf,
-                    const float* wsum, float* ujet) {
+                    const float* wsum, const float* A, const float* B, float* ujet) {
   int J, rc;
   if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  if ((A == nullptr) != (B == nullptr)) {
+    set_error("pou_reduce: A and B are both given or both NULL");
+    return FBPX_EINVAL;
+  }
   if (wl->n_points == 0) return FBPX_OK;
   hipStream_t s = (hipStream_t)stream;
   dim3 grid((wl->n_points + 255) / 256);
 #define X(XD, D1, D2)                                                                                      \
   if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                             \
-    hipLaunchKernelGGL((pou_reduce_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, pairbuf, wsum, ujet);
+    hipLaunchKernelGGL((pou_reduce_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, pairbuf, wsum, A, B, \
+                       ujet);
   FBPX_SCHEMAS(X)
 #undef X
   return hip_check(hipGetLastError(), "pou_reduce");
 }
 
 int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* wsum,
-                      const float* g_ujet, float* gseg) {
+                      const float* B, const float* g_ujet, float* gseg) {
   int J, rc;
   if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
   if (wl->n_points == 0) return FBPX_OK;
@@ -742,7 +809,8 @@ int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl
   dim3 grid((wl->n_points + 255) / 256);
 #define X(XD, D1, D2)                                                                                          \
   if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                                 \
-    hipLaunchKernelGGL((pou_backward_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, wsum, g_ujet, gseg);
+    hipLaunchKernelGGL((pou_backward_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, wsum, B, g_ujet, \
+                       gseg);
   FBPX_SCHEMAS(X)
 #undef X
   return hip_check(hipGetLastError(), "pou_backward");

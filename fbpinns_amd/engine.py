@@ -272,9 +272,8 @@ class FusedEngine:
         (loss, d loss/d theta [m][P], d loss/d pflat [Q] or None)."""
         plan = self.plan
         th = self.theta.detach().requires_grad_(True)
-        ujet = FBPINNJets.apply(th, plan)
         params, pleaf = self._params_for_step()
-        loss = self.epilogue.loss(ujet, params)
+        loss = self.epilogue.jets_loss(th, plan, params)
         leaves = [th] + ([pleaf] if pleaf is not None else [])
         grads = torch.autograd.grad(loss, leaves, allow_unused=True)
         g_p = None
@@ -285,9 +284,8 @@ class FusedEngine:
     def _step_body(self):
         plan = self.plan
         th = self.theta.detach().requires_grad_(True)
-        ujet = FBPINNJets.apply(th, plan)
         params, pleaf = self._params_for_step()
-        loss = self.epilogue.loss(ujet, params)
+        loss = self.epilogue.jets_loss(th, plan, params)
         # the preallocated seed d loss / d loss = 1 keeps a fill kernel out of the step
         if pleaf is not None:
             g_theta, g_p = torch.autograd.grad(loss, [th, pleaf], grad_outputs=(self.loss_seed,),

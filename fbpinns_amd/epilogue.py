@@ -27,8 +27,9 @@ description agrees with loss_fn at set-up, one fused kernel pair per residual
 import torch
 from torch.func import jvp
 
+from fbpinns_amd import _lib, ops
 from fbpinns_amd._lib import FBPX_MAX_REQ
-from fbpinns_amd.ops import AffineJets, FbpxError, GatherRows, ResidualLoss
+from fbpinns_amd.ops import AffineJets, FBPINNJets, FbpxError, GatherRows, ResidualLoss
 from fbpinns_amd.residuals import F, J, P, V, check_spec, spec_loss
 from fbpinns_amd.util.logger import logger
 
@@ -133,6 +134,10 @@ class Epilogue:
         self.col = _col_index(net)
         self.affine = self._affine_precompute(all_params)
         self.fused = self._fused_precompute(all_params) if fused_loss else None
+        self.combined = self._combined_setup()
+        # d loss / d (constrained jets): columns no residual reads stay 0
+        self.gC = (torch.zeros((net.J, x.shape[0]), dtype=torch.float32, device=x.device)
+                   if self.combined is not None else None)
 
     # -------------------------------------------------------------- set-up
 
@@ -298,6 +303,53 @@ class Epilogue:
                 return False
         return True
 
+    @staticmethod
+    def _coef(static, dyn, all_params):
+        if not dyn:
+            return static
+        parts = list(static.unbind())
+        for t, c in dyn:
+            parts[t] = torch.as_tensor(c(all_params)).reshape(()).to(torch.float32)
+        return torch.stack(parts)
+
+    def _combined_setup(self):
+        """Jet column of every derivative key when the jets kernel, the affine
+        constraining function and the residuals can run as one autograd node
+        (_JetsResidualLoss): affine constraint, fused residuals without
+        gathered trainable columns, every (constraint, jet column) read by
+        one residual only (its gradient is written, not accumulated).
+        Returns the per-residual column lists or None."""
+        if self.affine is None or self.fused is None:
+            return None
+        cols, seen = [], set()
+        for ic, weight, keys, data, fa, fb, static, dyn, done in self.fused:
+            cs = []
+            for k in keys:
+                if k[0] == "P":
+                    return None
+                if k[0] == "J":
+                    iu, ixs = self.required_ujss[ic][k[1]]
+                    c = (self.col[("v", None)] if len(ixs) == 0 else
+                         self.col[("d1", ixs[0])] if len(ixs) == 1 else self.col[("d2", ixs[0])])
+                    if (ic, c) in seen:
+                        return None
+                    seen.add((ic, c))
+                    cs.append(c)
+                else:
+                    cs.append(None)
+            cols.append(cs)
+        return cols
+
+    def jets_loss(self, theta, plan, all_params):
+        """The loss from the subdomain rows theta on the plan's points
+        (FBPINN_loss, trainers.py:249-267): one autograd node for jets,
+        constraint and residuals where _combined_setup allows, else the jets
+        (ops.FBPINNJets) followed by Epilogue.loss."""
+        if self.combined is not None:
+            coefs = [self._coef(static, dyn, all_params) for _, _, _, _, _, _, static, dyn, _ in self.fused]
+            return _JetsResidualLoss.apply(theta, plan, self, *coefs)
+        return self.loss(FBPINNJets.apply(theta, plan), all_params)
+
     def _fused_loss(self, ujss, all_params):
         loss = None
         for ic, weight, keys, data, fa, fb, static, dyn, done in self.fused:
@@ -311,12 +363,7 @@ class Epilogue:
                     cols.append(GatherRows.apply(v, idx, perm, seg))
                 else:
                     cols.append(data[k])
-            coef = static
-            if dyn:
-                parts = list(static.unbind())
-                for t, c in dyn:
-                    parts[t] = torch.as_tensor(c(all_params)).reshape(()).to(torch.float32)
-                coef = torch.stack(parts)
+            coef = self._coef(static, dyn, all_params)
             term = ResidualLoss.apply(coef, weight, fa, fb, done, *cols)
             loss = term if loss is None else loss + term
         return loss
@@ -399,3 +446,65 @@ def _with_grad_trainable(all_params):
             return d.detach().double().requires_grad_(True)
         return d
     return {"static": all_params["static"], "trainable": rec(all_params["trainable"])}
+
+
+class _JetsResidualLoss(torch.autograd.Function):
+    """Loss of the fused residual description from the subdomain rows, as one
+    autograd node (Epilogue.jets_loss): fbpx_forward -> fbpx_pou_reduce with
+    the affine constraining function applied to every jet column ->
+    fbpx_residual_loss per residual; backward: fbpx_residual_loss_backward
+    writing d loss / d (constrained jets) straight into the epilogue's
+    [J][n] buffer -> fbpx_pou_backward with the transposed affine map ->
+    fbpx_backward -> fbpx_reduce_rows.  Same kernels and arithmetic as
+    FBPINNJets + AffineJets + ResidualLoss, two launches fewer per step and
+    no per-column gradient tensors."""
+
+    @staticmethod
+    def forward(ctx, theta, plan, ep, *coefs):
+        net, wl = plan.net, plan.wl
+        ops.forward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
+        A, B = ep.affine
+        C = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=theta.device)
+        ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
+        total, rs = None, []
+        for (ic, weight, keys, data, fa, fb, static, dyn, done), cs, coef in zip(ep.fused, ep.combined, coefs):
+            off, n_c = ep.cranges[ic]
+            cols = [C[c, off:off + n_c] if c is not None else data[k] for c, k in zip(cs, keys)]
+            r = torch.empty(n_c, dtype=torch.float32, device=theta.device)
+            scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
+            loss = torch.empty((), dtype=torch.float32, device=theta.device)
+            ops.residual_loss(cols, list(fa), list(fb), coef, weight, r, scratch, loss, done)
+            rs.append(r)
+            total = loss if total is None else total + loss
+        ctx.save_for_backward(theta, C, *coefs, *rs)
+        ctx.plan, ctx.ep, ctx.nr = plan, ep, len(coefs)
+        return total
+
+    @staticmethod
+    def backward(ctx, g):
+        theta, C, *rest = ctx.saved_tensors
+        plan, ep, nr = ctx.plan, ctx.ep, ctx.nr
+        coefs, rs = rest[:nr], rest[nr:]
+        net, wl = plan.net, plan.wl
+        g = g.contiguous()
+        gcoefs = []
+        for i, ((ic, weight, keys, data, fa, fb, static, dyn, done), cs) in enumerate(zip(ep.fused, ep.combined)):
+            off, n_c = ep.cranges[ic]
+            cols = [C[c, off:off + n_c] if c is not None else data[k] for c, k in zip(cs, keys)]
+            gidx = [j for j, c in enumerate(cs) if c is not None]
+            gcols = [ep.gC[cs[j], off:off + n_c] for j in gidx]
+            want = bool(ctx.needs_input_grad[3 + i])
+            nt = len(fa)
+            gcoef = torch.empty(nt, dtype=torch.float32, device=g.device)
+            scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=g.device)
+            ops.residual_loss_backward(cols, list(fa), list(fb), coefs[i], rs[i], weight, g, gidx, gcols, scratch,
+                                       gcoef, want, done)
+            gcoefs.append(gcoef if want else None)
+        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1])
+        ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                     plan.gseg, plan.partial)
+        g_theta = plan.g_theta
+        if g_theta is None or g_theta.shape != theta.shape:
+            g_theta = torch.zeros_like(theta)
+        ops.reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
+        return (g_theta, None, None, *gcoefs)

@@ -11,7 +11,7 @@ kernel (activation recompute) + deterministic per-subdomain row reduction.
 """
 
 import ctypes
-from typing import List
+from typing import List, Optional
 
 import torch
 
@@ -172,22 +172,30 @@ def forward(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Ten
 
 @torch.library.custom_op("fbpx::pou_reduce", mutates_args=("ujet",))
 def pou_reduce(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf: torch.Tensor,
-               wsum: torch.Tensor, ujet: torch.Tensor) -> None:
+               wsum: torch.Tensor, ujet: torch.Tensor, A: Optional[torch.Tensor] = None,
+               B: Optional[torch.Tensor] = None) -> None:
+    """A, B: [J][n_points] jets of an affine constraining function (applied
+    to the reduced jets), or None."""
     n = _net_from_list(net)
     w = _wl_struct(wl_t, wl_n)
+    for t, nm in ((A, "A"), (B, "B")):
+        if t is not None and t.shape != ujet.shape:
+            raise FbpxError(f"pou_reduce: {nm} {tuple(t.shape)} for jets {tuple(ujet.shape)}")
     check(lib().fbpx_pou_reduce(_stream(), ctypes.byref(n), ctypes.byref(w), _p(pairbuf, torch.float32, "pairbuf"),
-                                _p(wsum, torch.float32, "wsum"), _p(ujet, torch.float32, "ujet")),
-          "fbpx_pou_reduce")
+                                _p(wsum, torch.float32, "wsum"), _p(A, torch.float32, "A"), _p(B, torch.float32, "B"),
+                                _p(ujet, torch.float32, "ujet")), "fbpx_pou_reduce")
 
 
 @torch.library.custom_op("fbpx::pou_backward", mutates_args=("gseg",))
 def pou_backward(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum: torch.Tensor,
-                 g_ujet: torch.Tensor, gseg: torch.Tensor) -> None:
+                 g_ujet: torch.Tensor, gseg: torch.Tensor, B: Optional[torch.Tensor] = None) -> None:
     n = _net_from_list(net)
     w = _wl_struct(wl_t, wl_n)
+    if B is not None and B.shape != g_ujet.shape:
+        raise FbpxError(f"pou_backward: B {tuple(B.shape)} for jets {tuple(g_ujet.shape)}")
     check(lib().fbpx_pou_backward(_stream(), ctypes.byref(n), ctypes.byref(w), _p(wsum, torch.float32, "wsum"),
-                                  _p(g_ujet, torch.float32, "g_ujet"), _p(gseg, torch.float32, "gseg")),
-          "fbpx_pou_backward")
+                                  _p(B, torch.float32, "B"), _p(g_ujet, torch.float32, "g_ujet"),
+                                  _p(gseg, torch.float32, "gseg")), "fbpx_pou_backward")
 
 
 @torch.library.custom_op("fbpx::backward", mutates_args=("partial",))

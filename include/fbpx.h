@@ -131,16 +131,22 @@ int fbpx_forward(void* stream, const fbpx_net* net, const float* theta,
 
 /* POU reduction: per point, sum over pou of (sum_pairs u*w) / (sum_pairs w)
  * with the quotient rule on jets, divided by npou (trainers.py:162-170).
- * Sorted, atomic-free.  ujet: [J][n_points] (column-major jets). */
+ * Sorted, atomic-free.  ujet: [J][n_points] (column-major jets).  A, B: NULL,
+ * or the [J][n_points] jets of an affine constraining_fn A + B u (see
+ * fbpx_affine_jets), which is then applied to every column of every point
+ * (constraining_fn inside the model, trainers.py:174) before ujet is written. */
 int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
-                    const float* pairbuf, const float* wsum, float* ujet);
+                    const float* pairbuf, const float* wsum, const float* A, const float* B,
+                    float* ujet);
 
 /* ---------------------------------------------------------------- backward */
 
 /* Transpose of fbpx_pou_reduce: from d loss / d ujet ([J][n_points]) to
- * d loss / d (segment sum of u*w jets), gseg: [n_segs][J]. */
+ * d loss / d (segment sum of u*w jets), gseg: [n_segs][J].  B: NULL, or the
+ * affine map's B jets as given to fbpx_pou_reduce (its transpose is applied
+ * first). */
 int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
-                      const float* wsum, const float* g_ujet, float* gseg);
+                      const float* wsum, const float* B, const float* g_ujet, float* gseg);
 
 /* Reverse pass through the jets to the subdomain weights (the part of
  * value_and_grad, trainers.py:292, below the segment sums).  Recomputes
