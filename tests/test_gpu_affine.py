@@ -75,3 +75,63 @@ def test_affine_jets_match_nested_jvp(prob, kw, xd, reqs):
     e = ((gq.cpu().double() - gq64).abs().max() / gq64.abs().max()).item()
     print(f"  backward rel err {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+@pytest.mark.parametrize("prob,kw,xd,lo,hi,nsub,ls", [
+    (P.WaveEquationConstantVelocity3D, dict(c0=1.0, source=P.wave_sources()), 3, (-1.0, -1.0, 0.0),
+     (1.0, 1.0, 1.0), (3, 3, 2), [3, 64, 64, 1]),
+    (P.BurgersEquation2D, dict(), 2, (-1.0, 0.0), (1.0, 1.0), (4, 3), [2, 32, 32, 1]),
+], ids=["wave", "burgers"])
+def test_pou_reduce_with_affine_map_equals_separate_kernels(prob, kw, xd, lo, hi, nsub, ls):
+    """fbpx_pou_reduce with the affine jets A, B equals fbpx_pou_reduce
+    followed by fbpx_affine_jets on every column (bit for bit: same
+    operations), and fbpx_pou_backward with B equals fbpx_affine_jets_backward
+    followed by fbpx_pou_backward."""
+    import numpy as np
+    import setups
+    from fbpinns_amd import ops
+    from fbpinns_amd.engine import FusedEngine
+    from fbpinns_amd.networks import FCN
+    dec_cls, ap = setups.build(nsub, lo, hi, 1.9, ls, seed=0)
+    ap["static"]["problem"] = prob.init_params(**kw)[0]
+    eng = FusedEngine(prob, dec_cls, FCN, ap, device="cuda:0", graph=False)
+    x = setups.uniform_points(3000, lo, hi, seed=4).cuda()
+    full = (1 << xd) - 1
+    plan, _ = eng.make_plan(x, schema=pick_schema(xd, full, full))
+    net = plan.net
+    print(f"[affine pou] {prob.__name__} J={net.J} pairs={plan.wl.n_pairs}", flush=True)
+    req = [((0, ()),) + tuple((0, (i,)) for i in range(xd)) + tuple((0, (i, i)) for i in range(xd))]
+    ep = epilogue.Epilogue(prob, eng.all_params, x, [(0, x.shape[0])], [[x]], req, net)
+    assert ep.affine is not None
+    A, B = ep.affine
+    wl = plan.wl
+    ops.forward(net.as_list(), eng.theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
+    q = torch.empty((net.J, x.shape[0]), device="cuda")
+    ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, q)
+    C = torch.empty_like(q)
+    ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
+    cols = []   # the request code of every jet column, in column order
+    for c in range(net.J):
+        code = 0 if c == 0 else None
+        for i in range(xd):
+            if c == net.col1(i):
+                code = 1 + i
+            if (net.d2mask >> i) & 1 and c == net.col2(i):
+                code = 1 + xd + i
+        cols.append(code)
+    outs = [torch.empty((x.shape[0], 1), device="cuda") for _ in cols]
+    ops.affine_jets(net.as_list(), 0, x.shape[0], A, B, q, cols, outs)
+    for c, o in enumerate(outs):
+        assert torch.equal(C[c], o.reshape(-1)), c
+    # backward
+    gC = torch.randn(C.shape, generator=torch.Generator().manual_seed(1)).cuda()
+    gseg_a = torch.empty_like(plan.gseg)
+    ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC, gseg_a, B)
+    gq = torch.empty_like(q)
+    ops.affine_jets_backward(net.as_list(), 0, x.shape[0], B, cols, [gC[c].reshape(-1, 1).contiguous()
+                                                                     for c in range(net.J)], gq)
+    gseg_b = torch.empty_like(plan.gseg)
+    ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gq, gseg_b)
+    e = ((gseg_a - gseg_b).abs().max() / gseg_b.abs().max()).item()
+    print(f"  forward bit-exact, backward rel {e:.1e}", flush=True)
+    assert e < 1e-6
