@@ -315,18 +315,16 @@ class Epilogue:
     def _combined_setup(self):
         """Jet column of every derivative key when the jets kernel, the affine
         constraining function and the residuals can run as one autograd node
-        (_JetsResidualLoss): affine constraint, fused residuals without
-        gathered trainable columns, every (constraint, jet column) read by
-        one residual only (its gradient is written, not accumulated).
-        Returns the per-residual column lists or None."""
+        (_JetsResidualLoss): affine constraint, fused residuals, every
+        (constraint, jet column) read by one residual only (its gradient is
+        written, not accumulated).  Returns the per-residual column lists
+        (jet column, or None for data and gathered columns) or None."""
         if self.affine is None or self.fused is None:
             return None
         cols, seen = [], set()
         for ic, weight, keys, data, fa, fb, static, dyn, done in self.fused:
             cs = []
             for k in keys:
-                if k[0] == "P":
-                    return None
                 if k[0] == "J":
                     iu, ixs = self.required_ujss[ic][k[1]]
                     c = (self.col[("v", None)] if len(ixs) == 0 else
@@ -347,7 +345,10 @@ class Epilogue:
         (ops.FBPINNJets) followed by Epilogue.loss."""
         if self.combined is not None:
             coefs = [self._coef(static, dyn, all_params) for _, _, _, _, _, _, static, dyn, _ in self.fused]
-            return _JetsResidualLoss.apply(theta, plan, self, *coefs)
+            # the trainable vectors of the gathered columns, in key order
+            pvecs = [torch.as_tensor(data[k][0](all_params)).reshape(-1).to(torch.float32)
+                     for _, _, keys, data, *_ in self.fused for k in keys if k[0] == "P"]
+            return _JetsResidualLoss.apply(theta, plan, self, len(coefs), *coefs, *pvecs)
         return self.loss(FBPINNJets.apply(theta, plan), all_params)
 
     def _fused_loss(self, ujss, all_params):
@@ -460,46 +461,75 @@ class _JetsResidualLoss(torch.autograd.Function):
     no per-column gradient tensors."""
 
     @staticmethod
-    def forward(ctx, theta, plan, ep, *coefs):
+    def _columns(ep, C, i, gathered):
+        """Residual i's factor columns: jet columns of C, data columns, and
+        the gathered columns (consumed from the iterator `gathered`)."""
+        ic, weight, keys, data, *_ = ep.fused[i]
+        off, n_c = ep.cranges[ic]
+        return [C[c, off:off + n_c] if c is not None else (next(gathered) if k[0] == "P" else data[k])
+                for c, k in zip(ep.combined[i], keys)]
+
+    @staticmethod
+    def forward(ctx, theta, plan, ep, nr, *inputs):
+        coefs, pvecs = inputs[:nr], inputs[nr:]
         net, wl = plan.net, plan.wl
         ops.forward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
         A, B = ep.affine
         C = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=theta.device)
         ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
+        # gathered columns v[idx] of the trainable vectors, in key order
+        pk = [data[k] for _, _, keys, data, *_ in ep.fused for k in keys if k[0] == "P"]
+        gathered = [v.index_select(0, d[1]) for v, d in zip(pvecs, pk)]
+        it = iter(gathered)
         total, rs = None, []
-        for (ic, weight, keys, data, fa, fb, static, dyn, done), cs, coef in zip(ep.fused, ep.combined, coefs):
-            off, n_c = ep.cranges[ic]
-            cols = [C[c, off:off + n_c] if c is not None else data[k] for c, k in zip(cs, keys)]
-            r = torch.empty(n_c, dtype=torch.float32, device=theta.device)
+        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
+            cols = _JetsResidualLoss._columns(ep, C, i, it)
+            r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=theta.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
             loss = torch.empty((), dtype=torch.float32, device=theta.device)
-            ops.residual_loss(cols, list(fa), list(fb), coef, weight, r, scratch, loss, done)
+            ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, loss, done)
             rs.append(r)
             total = loss if total is None else total + loss
-        ctx.save_for_backward(theta, C, *coefs, *rs)
-        ctx.plan, ctx.ep, ctx.nr = plan, ep, len(coefs)
+        ctx.save_for_backward(theta, C, *coefs, *rs, *gathered)
+        ctx.plan, ctx.ep, ctx.nr, ctx.pk = plan, ep, nr, pk
+        ctx.nvec = [v.numel() for v in pvecs]
         return total
 
     @staticmethod
     def backward(ctx, g):
         theta, C, *rest = ctx.saved_tensors
-        plan, ep, nr = ctx.plan, ctx.ep, ctx.nr
-        coefs, rs = rest[:nr], rest[nr:]
+        plan, ep, nr, pk = ctx.plan, ctx.ep, ctx.nr, ctx.pk
+        coefs, rs, gathered = rest[:nr], rest[nr:2 * nr], rest[2 * nr:]
         net, wl = plan.net, plan.wl
         g = g.contiguous()
-        gcoefs = []
-        for i, ((ic, weight, keys, data, fa, fb, static, dyn, done), cs) in enumerate(zip(ep.fused, ep.combined)):
+        it = iter(gathered)
+        gcoefs, gp = [], []   # gp: per gathered column, its gradient buffer
+        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
             off, n_c = ep.cranges[ic]
-            cols = [C[c, off:off + n_c] if c is not None else data[k] for c, k in zip(cs, keys)]
-            gidx = [j for j, c in enumerate(cs) if c is not None]
-            gcols = [ep.gC[cs[j], off:off + n_c] for j in gidx]
-            want = bool(ctx.needs_input_grad[3 + i])
+            cs = ep.combined[i]
+            cols = _JetsResidualLoss._columns(ep, C, i, it)
+            gidx, gcols = [], []
+            for j, (c, k) in enumerate(zip(cs, keys)):
+                if c is not None:
+                    gidx.append(j)
+                    gcols.append(ep.gC[c, off:off + n_c])
+                elif k[0] == "P":
+                    gidx.append(j)
+                    gcols.append(torch.empty(n_c, dtype=torch.float32, device=g.device))
+                    gp.append(gcols[-1])
+            want = bool(ctx.needs_input_grad[4 + i])
             nt = len(fa)
             gcoef = torch.empty(nt, dtype=torch.float32, device=g.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=g.device)
             ops.residual_loss_backward(cols, list(fa), list(fb), coefs[i], rs[i], weight, g, gidx, gcols, scratch,
                                        gcoef, want, done)
             gcoefs.append(gcoef if want else None)
+        # transposes of the gathers: per trainable vector, its points summed in index order
+        gvecs = []
+        for d, gcol, nv in zip(pk, gp, ctx.nvec):
+            out = torch.empty(nv, dtype=torch.float32, device=g.device)
+            ops.segment_sum(d[3], d[2], gcol, out)
+            gvecs.append(out)
         ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1])
         ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                      plan.gseg, plan.partial)
@@ -507,4 +537,4 @@ class _JetsResidualLoss(torch.autograd.Function):
         if g_theta is None or g_theta.shape != theta.shape:
             g_theta = torch.zeros_like(theta)
         ops.reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
-        return (g_theta, None, None, *gcoefs)
+        return (g_theta, None, None, None, *gcoefs, *gvecs)
