@@ -395,7 +395,10 @@ __global__ void __launch_bounds__(256) reduce_rows_kernel(int P, const float* __
 // count_done; the last workgroup to arrive adds 1 to the count (every other
 // workgroup has read it by then) and resets count_done to 0 for the next
 // launch.  Nothing else is ordered against the count, so no fence is needed.
-constexpr int kAdamBlocks = 2048;
+// Launches of at most this many workgroups finish their own reduction (or
+// the step count) in the last workgroup; larger ones use a second launch,
+// which costs less than a fence and an atomic per workgroup.
+constexpr int kSingleLaunchBlocks = 256;
 
 __device__ inline void adam_count_step(int32_t* count, int32_t* count_done) {
   if (count_done != nullptr && threadIdx.x == 0 && atomicAdd(count_done, 1) == (int)gridDim.x - 1) {
@@ -849,13 +852,15 @@ int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, 
     set_error("adam_rows: null pointer");
     return FBPX_EINVAL;
   }
-  // grid-stride over at most kAdamBlocks workgroups: few arrivals on the
-  // count_done counter, and enough workgroups to fill the GPU
-  const int64_t total = (int64_t)n_rows * P, need = (total + 255) / 256;
-  const unsigned nb = (unsigned)(need < kAdamBlocks ? need : kAdamBlocks);
-  hipLaunchKernelGGL(adam_rows_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream,
-                     theta, mu, nu, P, grad, rows, n_rows, count, count_done, adam_const(lr, b1, b2, eps));
-  return hip_check(hipGetLastError(), "adam_rows");
+  // the count is advanced in this launch (an atomic per workgroup) for a
+  // few workgroups, by a second one-thread launch otherwise
+  const int64_t total = (int64_t)n_rows * P, nb = (total + 255) / 256;
+  const bool fold = count_done && nb <= kSingleLaunchBlocks;
+  hipLaunchKernelGGL(adam_rows_kernel, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, theta, mu, nu, P,
+                     grad, rows, n_rows, count, fold ? count_done : nullptr, adam_const(lr, b1, b2, eps));
+  int rc = hip_check(hipGetLastError(), "adam_rows");
+  if (rc == FBPX_OK && count_done && !fold) rc = fbpx_count_increment(stream, count);
+  return rc;
 }
 
 int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g, int32_t n, int32_t* count,
@@ -873,9 +878,13 @@ int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g,
     set_error("adam_flat: null pointer");
     return FBPX_EINVAL;
   }
-  hipLaunchKernelGGL(adam_flat_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, mu, nu, g, n,
-                     count, count_done, adam_const(lr, b1, b2, eps));
-  return hip_check(hipGetLastError(), "adam_flat");
+  const int nb = (n + 255) / 256;
+  const bool fold = count_done && nb <= kSingleLaunchBlocks;
+  hipLaunchKernelGGL(adam_flat_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, p, mu, nu, g, n, count,
+                     fold ? count_done : nullptr, adam_const(lr, b1, b2, eps));
+  int rc = hip_check(hipGetLastError(), "adam_flat");
+  if (rc == FBPX_OK && count_done && !fold) rc = fbpx_count_increment(stream, count);
+  return rc;
 }
 
 int fbpx_count_increment(void* stream, int32_t* count) {
@@ -1010,15 +1019,18 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
   const int nb = residual_blocks(n);
 #define FBPX_RES_CASE(NT)                                                                          \
   case NT:                                                                                         \
-    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, done); \
+    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, one); \
     break;
   const float scale = weight / (float)n;
+  // the last-workgroup sum (a fence and an atomic per workgroup) pays off
+  // against a second launch only for a few workgroups
+  int32_t* const one = nb <= kSingleLaunchBlocks ? done : nullptr;
   switch (n_terms) {
     FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
   }
 #undef FBPX_RES_CASE
-  if (!done) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss);
+  if (!one) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss);
   return hip_check(hipGetLastError(), "residual_loss");
 }
 
@@ -1039,14 +1051,15 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
 #define FBPX_RES_CASE(NT)                                                                                 \
   case NT:                                                                                                \
     hipLaunchKernelGGL(residual_backward_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, w2n, g_loss, \
-                       scratch, gcoef, gcoef ? done : nullptr);                                           \
+                       scratch, gcoef, one);                                                              \
     break;
+  int32_t* const one = gcoef && nb <= kSingleLaunchBlocks ? done : nullptr;
   switch (n_terms) {
     FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
   }
 #undef FBPX_RES_CASE
-  if (gcoef && !done)
+  if (gcoef && !one)
     hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
   return hip_check(hipGetLastError(), "residual_loss_backward");
 }

@@ -183,9 +183,11 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial,
  * theta, mu, nu, grad: [m][P].  The hyper-parameters are doubles so that
  * f32(1 - b1) and f32(-lr) are formed as the reference forms them (Python
  * floats meeting FP32 arrays).  count_done: NULL, or a DEVICE int32 that is 0
- * before the call (the call leaves it 0): then the same launch also does
- * *count += 1 once every workgroup has read the count (the step's last
- * Adam launch carries it, in place of fbpx_count_increment). */
+ * before the call (the call leaves it 0): then the call also does
+ * *count += 1 after every workgroup has read the count, inside the launch
+ * when it has at most 256 workgroups (its last workgroup), else by a second
+ * launch (the step's last Adam call carries it, in place of
+ * fbpx_count_increment). */
 int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu,
                    int32_t P, const float* grad, const int32_t* rows, int32_t n_rows,
                    int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps);
@@ -248,9 +250,10 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
  * on the device).  Writes r[n] and loss[0]; scratch: FBPX_RESIDUAL_BLOCKS floats.
  * done: NULL (a second one-workgroup launch sums the per-workgroup partials)
- * or a DEVICE int32 that is 0 before the call and is left 0: the last
- * workgroup of the one launch sums them.  Either way the sum has a fixed
- * order (deterministic, no memset).
+ * or a DEVICE int32 that is 0 before the call and is left 0: then a launch of
+ * at most 256 workgroups sums them in its last workgroup (one launch instead
+ * of two; larger grids keep the second launch).  Either way the sum has a
+ * fixed order (deterministic, no memset).
  */
 #define FBPX_MAX_TERMS 8
 #define FBPX_RESIDUAL_BLOCKS 1024
