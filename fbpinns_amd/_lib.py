@@ -58,6 +58,8 @@ SIGNATURES = {
     "fbpx_reduce_rows": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p]),
     "fbpx_adam_rows": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p,
                                 _c_d, _c_d, _c_d, _c_d]),
+    "fbpx_adam_rows_partial": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p,
+                                        _c_d, _c_d, _c_d, _c_d]),
     "fbpx_adam_flat": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_d, _c_d, _c_d, _c_d]),
     "fbpx_count_increment": (_c_i32, [_c_p, _c_p]),
     "fbpx_mean": (_c_i32, [_c_p, _c_p, ctypes.c_int64, _c_p, _c_p]),

@@ -407,9 +407,14 @@ __device__ inline void adam_count_step(int32_t* count, int32_t* count_done) {
   }
 }
 
+// grad != NULL: the gradient rows; else the per-chunk partial rows, summed
+// per listed row as fbpx_reduce_rows sums them (ascending chunks, same
+// unrolling), so that both forms give the same update bit for bit.
 __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ theta, float* __restrict__ mu,
                                                         float* __restrict__ nu, int P,
                                                         const float* __restrict__ grad,
+                                                        const float* __restrict__ partial,
+                                                        const int32_t* __restrict__ rcs,
                                                         const int32_t* __restrict__ rows, int n_rows,
                                                         int32_t* count, int32_t* count_done, AdamConst k) {
   // bias corrections once per workgroup
@@ -426,8 +431,16 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int row = (int)(idx / P), e = (int)(idx % P);
     const size_t i = (size_t)rows[row] * P + e;
+    float gi;
+    if (grad != nullptr) {
+      gi = grad[i];
+    } else {
+      gi = 0.f;
+#pragma unroll 4
+      for (int c = rcs[row]; c < rcs[row + 1]; ++c) gi += partial[(size_t)c * P + e];
+    }
     float p = theta[i], m = mu[i], v = nu[i];
-    adam_elem(p, m, v, grad[i], bc1, bc2, k);
+    adam_elem(p, m, v, gi, bc1, bc2, k);
     theta[i] = p;
     mu[i] = m;
     nu[i] = v;
@@ -836,9 +849,9 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial, const int32_
   return hip_check(hipGetLastError(), "reduce_rows");
 }
 
-int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
-                   const int32_t* rows, int32_t n_rows, int32_t* count, int32_t* count_done, double lr, double b1,
-                   double b2, double eps) {
+static int adam_rows_launch(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
+                            const float* partial, const int32_t* rcs, const int32_t* rows, int32_t n_rows,
+                            int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps) {
   if (P <= 0 || n_rows < 0) {
     set_error("adam_rows: bad sizes P=%d n_rows=%d", P, n_rows);
     return FBPX_EINVAL;
@@ -848,7 +861,7 @@ int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, 
     return FBPX_EINVAL;
   }
   if (n_rows == 0) return count_done ? fbpx_count_increment(stream, count) : FBPX_OK;
-  if (!theta || !mu || !nu || !grad || !rows) {
+  if (!theta || !mu || !nu || !rows || (!grad && (!partial || !rcs))) {
     set_error("adam_rows: null pointer");
     return FBPX_EINVAL;
   }
@@ -857,10 +870,33 @@ int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, 
   const int64_t total = (int64_t)n_rows * P, nb = (total + 255) / 256;
   const bool fold = count_done && nb <= kSingleLaunchBlocks;
   hipLaunchKernelGGL(adam_rows_kernel, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, theta, mu, nu, P,
-                     grad, rows, n_rows, count, fold ? count_done : nullptr, adam_const(lr, b1, b2, eps));
+                     grad, partial, rcs, rows, n_rows, count, fold ? count_done : nullptr,
+                     adam_const(lr, b1, b2, eps));
   int rc = hip_check(hipGetLastError(), "adam_rows");
   if (rc == FBPX_OK && count_done && !fold) rc = fbpx_count_increment(stream, count);
   return rc;
+}
+
+int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
+                   const int32_t* rows, int32_t n_rows, int32_t* count, int32_t* count_done, double lr, double b1,
+                   double b2, double eps) {
+  if (!grad && n_rows > 0) {
+    set_error("adam_rows: null grad");
+    return FBPX_EINVAL;
+  }
+  return adam_rows_launch(stream, theta, mu, nu, P, grad, nullptr, nullptr, rows, n_rows, count, count_done, lr, b1,
+                          b2, eps);
+}
+
+int fbpx_adam_rows_partial(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* partial,
+                           const int32_t* rows, const int32_t* row_chunk_start, int32_t n_rows, int32_t* count,
+                           int32_t* count_done, double lr, double b1, double b2, double eps) {
+  if ((!partial || !row_chunk_start) && n_rows > 0) {
+    set_error("adam_rows_partial: null partial / row_chunk_start");
+    return FBPX_EINVAL;
+  }
+  return adam_rows_launch(stream, theta, mu, nu, P, nullptr, partial, row_chunk_start, rows, n_rows, count,
+                          count_done, lr, b1, b2, eps);
 }
 
 int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g, int32_t n, int32_t* count,

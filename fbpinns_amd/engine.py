@@ -271,6 +271,7 @@ class FusedEngine:
         """Loss and gradients at the current parameters without an update:
         (loss, d loss/d theta [m][P], d loss/d pflat [Q] or None)."""
         plan = self.plan
+        plan.defer_rows = False
         th = self.theta.detach().requires_grad_(True)
         params, pleaf = self._params_for_step()
         loss = self.epilogue.jets_loss(th, plan, params)
@@ -283,19 +284,19 @@ class FusedEngine:
 
     def _step_body(self):
         plan = self.plan
+        # the backward leaves the per-chunk gradient rows in plan.partial;
+        # adam_rows_partial sums and applies them in one launch
+        plan.defer_rows = True
         th = self.theta.detach().requires_grad_(True)
         params, pleaf = self._params_for_step()
         loss = self.epilogue.jets_loss(th, plan, params)
         # the preallocated seed d loss / d loss = 1 keeps a fill kernel out of the step
-        if pleaf is not None:
-            g_theta, g_p = torch.autograd.grad(loss, [th, pleaf], grad_outputs=(self.loss_seed,),
-                                               allow_unused=True)
-        else:
-            (g_theta,) = torch.autograd.grad(loss, [th], grad_outputs=(self.loss_seed,))
-            g_p = None
+        leaves = [th] + ([pleaf] if pleaf is not None else [])
+        grads = torch.autograd.grad(loss, leaves, grad_outputs=(self.loss_seed,), allow_unused=True)
+        g_p = grads[1] if pleaf is not None else None
         # the step's last Adam launch also advances the shared count
-        ops.adam_rows(self.theta, self.mu, self.nu, g_theta, plan.rows, self.count, self.count_done, not self.Q,
-                      self.lr, self.b1, self.b2, self.eps)
+        ops.adam_rows_partial(self.theta, self.mu, self.nu, plan.partial, plan.rows, plan.row_chunk_start,
+                              self.count, self.count_done, not self.Q, self.lr, self.b1, self.b2, self.eps)
         if self.Q:
             if g_p is None:
                 g_p = torch.zeros_like(self.pflat)

@@ -533,8 +533,4 @@ class _JetsResidualLoss(torch.autograd.Function):
         ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1])
         ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                      plan.gseg, plan.partial)
-        g_theta = plan.g_theta
-        if g_theta is None or g_theta.shape != theta.shape:
-            g_theta = torch.zeros_like(theta)
-        ops.reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
-        return (g_theta, None, None, None, *gcoefs, *gvecs)
+        return (ops.theta_gradient(plan, theta), None, None, None, *gcoefs, *gvecs)

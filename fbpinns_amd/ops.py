@@ -232,6 +232,24 @@ def adam_rows(theta: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, grad: tor
           "fbpx_adam_rows")
 
 
+@torch.library.custom_op("fbpx::adam_rows_partial", mutates_args=("theta", "mu", "nu", "count", "count_done"))
+def adam_rows_partial(theta: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, partial: torch.Tensor,
+                      rows: torch.Tensor, row_chunk_start: torch.Tensor, count: torch.Tensor,
+                      count_done: torch.Tensor, advance: bool, lr: float, b1: float, b2: float, eps: float) -> None:
+    """reduce_rows + adam_rows in one launch (the gradient never materialised)."""
+    P = theta.shape[1]
+    if partial.shape[1] != P or row_chunk_start.numel() != rows.numel() + 1:
+        raise FbpxError(f"adam_rows_partial: partial {tuple(partial.shape)}, {rows.numel()} rows, "
+                        f"{row_chunk_start.numel()} row offsets")
+    check(lib().fbpx_adam_rows_partial(_stream(), _p(theta, torch.float32, "theta"), _p(mu, torch.float32, "mu"),
+                                       _p(nu, torch.float32, "nu"), P, _p(partial, torch.float32, "partial"),
+                                       _p(rows, torch.int32, "rows"),
+                                       _p(row_chunk_start, torch.int32, "row_chunk_start"), rows.shape[0],
+                                       _p(count, torch.int32, "count"),
+                                       _p(count_done, torch.int32, "count_done") if advance else None, lr, b1, b2,
+                                       eps), "fbpx_adam_rows_partial")
+
+
 @torch.library.custom_op("fbpx::adam_flat", mutates_args=("p", "mu", "nu", "count", "count_done"))
 def adam_flat(p: torch.Tensor, mu: torch.Tensor, nu: torch.Tensor, g: torch.Tensor, count: torch.Tensor,
               count_done: torch.Tensor, advance: bool, lr: float, b1: float, b2: float, eps: float) -> None:
@@ -520,9 +538,24 @@ class Plan:
         # [m][P] buffer every time (untrained rows stay 0, so no zero fill per
         # step); callers that keep a gradient across steps copy it
         self.g_theta = None
+        # set by the engine's step: the backward leaves the per-chunk rows in
+        # `partial` for adam_rows_partial and returns no theta gradient
+        self.defer_rows = False
 
     def keep_gradient_buffer(self, P):
         self.g_theta = torch.zeros((self.m, P), dtype=torch.float32, device=self.x.device)
+
+
+def theta_gradient(plan, theta):
+    """d loss / d theta from the per-chunk rows in plan.partial, or None when
+    the plan defers the row sums to adam_rows_partial."""
+    if plan.defer_rows:
+        return None
+    g_theta = plan.g_theta
+    if g_theta is None or g_theta.shape != theta.shape:
+        g_theta = torch.zeros_like(theta)
+    reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
+    return g_theta
 
 
 class FBPINNJets(torch.autograd.Function):
@@ -550,8 +583,4 @@ class FBPINNJets(torch.autograd.Function):
         pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, g_ujet, plan.gseg)
         backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                  plan.gseg, plan.partial)
-        g_theta = plan.g_theta
-        if g_theta is None or g_theta.shape != theta.shape:
-            g_theta = torch.zeros_like(theta)
-        reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g_theta)
-        return g_theta, None
+        return theta_gradient(plan, theta), None

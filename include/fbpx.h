@@ -192,6 +192,16 @@ int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu,
                    int32_t P, const float* grad, const int32_t* rows, int32_t n_rows,
                    int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps);
 
+/* fbpx_reduce_rows followed by fbpx_adam_rows in one launch: the gradient of
+ * each listed row is summed from the per-chunk partial rows (partial,
+ * row_chunk_start as for fbpx_reduce_rows, same order) and applied at once,
+ * without a gradient array.  Same results bit for bit.  No counterpart in the
+ * reference beyond the two steps it fuses (trainers.py:292-295). */
+int fbpx_adam_rows_partial(void* stream, float* theta, float* mu, float* nu, int32_t P,
+                           const float* partial, const int32_t* rows, const int32_t* row_chunk_start,
+                           int32_t n_rows, int32_t* count, int32_t* count_done, double lr, double b1,
+                           double b2, double eps);
+
 /* The same update on a flat vector p[n] with gradient g[n] (the non-subdomain
  * trainable leaves, e.g. a learnable PDE coefficient, problems.py:232-234). */
 int fbpx_adam_flat(void* stream, float* p, float* mu, float* nu, const float* g,

@@ -289,3 +289,35 @@ def test_fused_residual_loss_steps_match_loss_fn(name):
     Wb = eb.all_params["trainable"]["network"]["subdomain"]["layers"][0][0]
     assert (W - Wb).abs().max().item() <= 1e-5
     assert torch.allclose(ea.pflat, eb.pflat, rtol=1e-5, atol=1e-7)
+
+
+def test_adam_rows_partial_equals_reduce_then_adam():
+    """fbpx_adam_rows_partial (row sums and Adam in one launch) equals
+    fbpx_reduce_rows followed by fbpx_adam_rows, bit for bit, and advances the
+    shared count once either way."""
+    from fbpinns_amd import ops
+    eng, *_ = _build("wave_small")
+    plan = eng.plan
+    plan.defer_rows = False
+    eng.loss_and_grad()            # fills plan.partial with this step's chunk rows
+    torch.cuda.synchronize()
+    th0, mu0, nu0 = eng.theta.clone(), eng.mu.clone(), eng.nu.clone()
+    args = (eng.lr, eng.b1, eng.b2, eng.eps)
+    out = []
+    for fused in (False, True):
+        th, mu, nu = th0.clone(), mu0.clone(), nu0.clone()
+        count = torch.zeros(1, dtype=torch.int32, device="cuda")
+        done = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if fused:
+            ops.adam_rows_partial(th, mu, nu, plan.partial, plan.rows, plan.row_chunk_start, count, done, True,
+                                  *args)
+        else:
+            g = torch.zeros_like(th)
+            ops.reduce_rows(plan.partial, plan.rows, plan.row_chunk_start, g)
+            ops.adam_rows(th, mu, nu, g, plan.rows, count, done, True, *args)
+        torch.cuda.synchronize()
+        assert int(count.item()) == 1 and int(done.item()) == 0
+        out.append((th, mu, nu))
+    print(f"[adam partial] rows={plan.rows.numel()} P={eng.net.P}", flush=True)
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
