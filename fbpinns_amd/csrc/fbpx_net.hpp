@@ -831,19 +831,25 @@ struct DW2Own {
 // rebuilt from h: KIND 0: h, 1: (1 - h^2) sc, 2: h (1 - h^2) sc with sc
 // supplied by the caller (zt_i and -2 zt_i^2 for unit u).
 // stgl[ks] = stg + (NRB == 2 ? row block (wave >> 1) * BS : 0) +
-// chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3) and stbl = stb + ibw * BS +
-// 16 g + pl are this lane's read bases for pair 4 ks + g, unit pl (source
-// wave w at + w * WS, row block ob at + ob * BS; stb pair group ks at
-// + ks * PG).
-template <int H, int KIND>
-__device__ inline void dw2_shared(const float* const* stgl, const float* stbl, float sc, f4* dW2, int wave) {
+// chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3) is this lane's G read base for
+// pair 4 ks + g, unit pl (source wave w at + w * WS, row block ob at
+// + ob * BS); its h read base is stbl + ks * PG with stbl = stb + ibw * BS +
+// 16 g + pl (plain layout) or stgl[ks] + db (swizzled: both buffers share
+// the layout, db = stb - stg + the row-block offsets).
+template <int H, int KIND, bool SWZ>
+__device__ inline void dw2_shared(const float* const* stgl, const float* stbl, int db, float sc, f4* dW2,
+                                  int wave) {
   using O = DW2Own<H>;
+  // swizzled buffers: the h buffer's read base for pair group ks is
+  // stgl[ks] + db; db is kept opaque so that the four sums are formed at
+  // their reads instead of held in four more registers through the tile
+  if (SWZ) asm volatile("" : "+v"(db));
 #pragma unroll
   for (int s = 0; s < O::NSRC; ++s) {
     const int w = O::NSRC == 1 ? wave : s;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      const float h = stbl[w * O::WS + ks * O::PG];
+      const float h = SWZ ? stgl[ks][db + w * O::WS] : stbl[w * O::WS + ks * O::PG];
       float b = h;
       if (KIND == 1) b = fmaf(-h, h, 1.f) * sc;
       if (KIND == 2) b = h * fmaf(-h, h, 1.f) * sc;
@@ -912,13 +918,16 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   const int ucol = 16 * O::col_block(wave) + pl;   // this lane's dW2 column unit
   const float wl_red = ss[SM::WL + 16 * (pl >> 2) + 4 * g + (pl & 3)];   // WL of the unit a reduction leaves here
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
-  // the G buffer (five stores per tile) is swizzled at h = 64 where the four
-  // extra read bases fit the register budget (every first-derivative axis
-  // also carries its second derivative)
+  // both transpose buffers are swizzled at h = 64 where the four extra read
+  // bases fit the register budget (every first-derivative axis also carries
+  // its second derivative; the h buffer's reads add an offset to them)
   constexpr bool SWZ = H == 64 && D1M == D2M;
-  float* const stbw = stb + wave * O::WS + O::template chunk_off<false>(pl >> 2, pl & 3, g);   // store bases
+  float* const stbw = stb + wave * O::WS + O::template chunk_off<SWZ>(pl >> 2, pl & 3, g);   // store bases
   float* const stgw = stg + wave * O::WS + O::template chunk_off<SWZ>(pl >> 2, pl & 3, g);
-  const float* const stbl = stb + O::col_block(wave) * O::BS + 16 * g + pl;   // dW2 read bases
+  // dW2 read bases: stgl[ks] for the G buffer; for the h buffer stbl (plain
+  // layout) or stgl[ks] + stb_off (swizzled)
+  const float* const stbl = stb + O::col_block(wave) * O::BS + 16 * g + pl;
+  const int stb_off = (int)(stb - stg) + O::col_block(wave) * O::BS - (H == 32 ? (wave >> 1) * O::BS : 0);
   const float* stgl[4];
 #pragma unroll
   for (int ks = 0; ks < 4; ++ks)
@@ -1033,7 +1042,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
         for (int r = 0; r < 4; ++r) gz10[rb][r] = vt[rb][r] * rv[rb][r];
       __syncthreads();
-      dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
+      dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
       __syncthreads();
     }
     // layer-1 h for the B operands of the remaining dW2 columns (published
@@ -1105,7 +1114,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
             }
         }
         __syncthreads();
-        dw2_shared<H, 1>(stgl, stbl, szt[zrow + ucol], dW2, wave);
+        dw2_shared<H, 1, SWZ>(stgl, stbl, stb_off, szt[zrow + ucol], dW2, wave);
         __syncthreads();
       } else {
 #pragma unroll
@@ -1167,7 +1176,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 #pragma unroll
     for (int c = 0; c < J; ++c) gUn[c] = valid_n ? gseg[(size_t)segn * J + c] : 0.f;
     __syncthreads();
-    dw2_shared<H, 0>(stgl, stbl, 1.f, dW2, wave);
+    dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
     // (the next tile's leading barrier orders these reads before new writes)
   }
 
