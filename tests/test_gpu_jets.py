@@ -60,10 +60,8 @@ def test_takes_bit_exact(case):
     assert takes[4] == otakes[4]
 
 
-@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_forward_jets(case):
-    eng, ap, x, xd, d1, d2 = _setup(case)
-    print(f"[fwd] {case[0]} n={x.shape[0]} schema=({d1},{d2})", flush=True)
+def _check_forward(eng, ap, x, xd, d1, d2, label):
+    print(f"[fwd] {label} n={x.shape[0]} schema=({d1},{d2})", flush=True)
     plan, _ = eng.make_plan(x.cuda(), schema=(d1, d2))
     with torch.no_grad():
         ujet = FBPINNJets.apply(eng.theta, plan)
@@ -75,10 +73,11 @@ def test_forward_jets(case):
     assert (err <= JET_TOL).all(), err
 
 
-@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_backward_grads(case):
-    eng, ap, x, xd, d1, d2 = _setup(case, seed=3)
-    print(f"[bwd] {case[0]} n={x.shape[0]} schema=({d1},{d2})", flush=True)
+def _check_backward(eng, ap, x, xd, d1, d2, label):
+    """Synthetic loss sum(ujet * r) with fixed random r; gradients per layer
+    tensor against float64 autograd through the oracle's nested jvp; models
+    without pairs get exactly zero."""
+    print(f"[bwd] {label} n={x.shape[0]} schema=({d1},{d2})", flush=True)
     plan, _ = eng.make_plan(x.cuda(), schema=(d1, d2))
     J, n = plan.net.J, x.shape[0]
     r = torch.tensor(np.random.default_rng(5).standard_normal((J, n)), dtype=torch.float32)
@@ -86,7 +85,6 @@ def test_backward_grads(case):
     ujet = FBPINNJets.apply(th, plan)
     (ujet * r.cuda()).sum().backward()
     g = th.grad.detach().cpu()
-    # oracle: same synthetic loss, float64 autograd through the nested jvp
     takes, all_ims, layers, dom, _ = setups.oracle_view(ap, x)
     jm = om.get_jmaps(setups.schema_request(xd, d1, d2))
     layers = [(W.requires_grad_(True), b.requires_grad_(True)) for W, b in layers]
@@ -107,6 +105,49 @@ def test_backward_grads(case):
             assert e <= GRAD_TOL, (nm, e)
     unused = np.setdiff1d(np.arange(eng.m), all_ims)
     assert (g[unused] == 0).all()
+    return len(all_ims), eng.m
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_forward_jets(case):
+    _check_forward(*_setup(case), case[0])
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_backward_grads(case):
+    _check_backward(*_setup(case, seed=3), case[0])
+
+
+# Ragged work lists: points over part of the domain only (whole subdomains
+# without pairs, partial tiles at every chunk edge) and a handful of points
+# (nearly every tile row empty).  name, case index, n points, fraction of the
+# first axis the points cover
+RAGGED = [
+    ("poisson_left_strip", 2, 700, 0.3),
+    ("wave_left_strip", 5, 900, 0.35),
+    ("wave_7_points", 5, 7, 1.0),
+    ("ho_3_points", 0, 3, 1.0),
+    ("1d_h16_one_point", 1, 1, 1.0),
+]
+
+
+def _setup_ragged(idx, n, frac, seed):
+    name, nsub, lo, hi, ov, ls, (d1, d2), _, ml = CASES[idx]
+    dec_cls, ap = setups.build(nsub, lo, hi, ov, ls, seed=seed, multilevel=ml, unnorm=(0.1, 1.3))
+    hi_pts = (lo[0] + frac * (hi[0] - lo[0]),) + tuple(hi[1:])
+    x = setups.uniform_points(n, lo, hi_pts, seed=seed + 11)
+    eng = FusedEngine(Problem, dec_cls, __import__("fbpinns_amd.networks").networks.FCN, ap, device="cuda:0",
+                      graph=False)
+    return eng, ap, x, len(lo), d1, d2
+
+
+@pytest.mark.parametrize("rag", RAGGED, ids=[r[0] for r in RAGGED])
+def test_ragged_work_lists(rag):
+    label, idx, n, frac = rag
+    _check_forward(*_setup_ragged(idx, n, frac, seed=2), label)
+    n_used, m = _check_backward(*_setup_ragged(idx, n, frac, seed=4), label)
+    if frac < 1.0:
+        assert n_used < m        # some subdomains hold no points
 
 
 def test_fixed_subdomains_get_no_gradient():
