@@ -187,7 +187,8 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial,
  * *count += 1 after every workgroup has read the count, inside the launch
  * when it has at most 256 workgroups (its last workgroup), else by a second
  * launch (the step's last Adam call carries it, in place of
- * fbpx_count_increment). */
+ * fbpx_count_increment).  Calls that may run at the same time (other
+ * streams) need distinct count_done counters. */
 int fbpx_adam_rows(void* stream, float* theta, float* mu, float* nu,
                    int32_t P, const float* grad, const int32_t* rows, int32_t n_rows,
                    int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps);
@@ -263,7 +264,8 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * or a DEVICE int32 that is 0 before the call and is left 0: then a launch of
  * at most 256 workgroups sums them in its last workgroup (one launch instead
  * of two; larger grids keep the second launch).  Either way the sum has a
- * fixed order (deterministic, no memset).
+ * fixed order (deterministic, no memset).  Calls that may run at the same
+ * time (other streams) need distinct done counters and scratch arrays.
  */
 #define FBPX_MAX_TERMS 8
 #define FBPX_RESIDUAL_BLOCKS 1024
