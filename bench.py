@@ -57,6 +57,33 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "launches": 5}))
 
 
+def train_l1(name):
+    """Train configuration `name` for its full step count through
+    FBPINNTrainer (trainers.py:577-687) and print one JSON line: test L1 and
+    normalised L1 against the problem's exact solution at every test step
+    (trainers.py:727-774), the steady-state rate, and for the batched inverse
+    configuration the learnt damping coefficients against the true ones."""
+    from fbpinns_amd import configs
+    from fbpinns_amd.trainers import FBPINNTrainer
+    from fbpinns_amd.util.logger import logger
+    for h in logger.handlers:
+        h.setStream(sys.stderr)
+    c = configs.CONFIGS[name]()
+    tr = FBPINNTrainer(c, write_outputs=False)
+    t0 = time.time()
+    ap = tr.train()
+    rec = {"config": name, "n_steps": c.n_steps, "wall_s": round(time.time() - t0, 2),
+           "test_points": int(np.prod(c.n_test)),
+           "l1": [{"step": int(r[0]), "l1": r[4], "l1_normalised": r[5]} for r in tr.u_test_losses]}
+    if "mu" in ap["trainable"].get("problem", {}):
+        mu = ap["trainable"]["problem"]["mu"].detach().double().cpu().numpy().reshape(-1)
+        mu_true = np.asarray(ap["static"]["problem"]["mu_true"], dtype=np.float64).reshape(-1)
+        rel = np.abs(mu - mu_true) / np.abs(mu_true)
+        rec["mu_rel_err"] = {"median": float(np.median(rel)), "mean": float(rel.mean()), "max": float(rel.max()),
+                             "instances": int(rel.size)}
+    print(json.dumps(rec), flush=True)
+
+
 def time_kernel(fn, reps):
     """Average duration (ms) of fn() launched on torch's current stream,
     measured with HIP events on that stream."""
@@ -194,6 +221,8 @@ def main():
                     help="keep the default launch shape (counter passes see one launch shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--train-l1", action="store_true",
+                    help="only train --config for its full step count and print its test L1 history")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every run here uses one MI355X)",
@@ -201,6 +230,9 @@ def main():
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.train_l1:
+        train_l1(a.config)
         return
 
     from fbpinns_amd import autotune, configs
