@@ -79,8 +79,12 @@ def train_l1(name):
         mu = ap["trainable"]["problem"]["mu"].detach().double().cpu().numpy().reshape(-1)
         mu_true = np.asarray(ap["static"]["problem"]["mu_true"], dtype=np.float64).reshape(-1)
         rel = np.abs(mu - mu_true) / np.abs(mu_true)
+        w0 = np.asarray(ap["static"]["problem"]["w0"], dtype=np.float64).reshape(-1)
+        worst = np.argsort(-rel)[:8]
         rec["mu_rel_err"] = {"median": float(np.median(rel)), "mean": float(rel.mean()), "max": float(rel.max()),
-                             "instances": int(rel.size)}
+                             "instances": int(rel.size), "within_5pct": int((rel < 0.05).sum()),
+                             "worst": [{"instance": int(i), "w0": float(w0[i]), "mu_true": float(mu_true[i]),
+                                        "mu": float(mu[i])} for i in worst]}
     print(json.dumps(rec), flush=True)
 
 
