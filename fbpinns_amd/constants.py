@@ -96,6 +96,7 @@ class Constants(ConstantsBase):
         self.graph = True          # capture the whole train step in a hipGraph
         self.autotune = True       # time candidate launch shapes per work-list shape
         self.fused_loss = True     # problem.residuals() -> fused residual-loss kernels (else loss_fn)
+        self.sort_points = True    # with a residual description: each constraint's points in Z-order
         self.device = "cuda:0"
         self.hostname = socket.gethostname().lower()
         for key in kwargs.keys():

@@ -25,7 +25,7 @@ from fbpinns_amd import autotune, ops
 from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
 from fbpinns_amd.epilogue import Epilogue, schema_masks
 from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
-from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch
+from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch, spatial_order
 from fbpinns_amd.util.logger import logger
 
 # compiled jet schemas per input dimension (fbpx_shapes.hpp)
@@ -78,7 +78,7 @@ class FusedEngine:
     """Owns the packed device state and runs train steps for one problem."""
 
     def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
-                 graph=True, autotune=True, fused_loss=True):
+                 graph=True, autotune=True, fused_loss=True, sort_points=True):
         from fbpinns_amd.decompositions import RectangularDecompositionND
         from fbpinns_amd.networks import FCN
         if not torch.cuda.is_available():
@@ -92,6 +92,7 @@ class FusedEngine:
         self.graph_enabled = bool(graph)
         self.autotune_enabled = bool(autotune)
         self.fused_loss = bool(fused_loss)
+        self.sort_points = bool(sort_points)
         ok = dict(optimiser_kwargs or {})
         self.lr = float(ok.pop("learning_rate", 1e-3))
         self.b1 = float(ok.pop("b1", 0.9))
@@ -183,6 +184,17 @@ class FusedEngine:
     def set_constraints(self, constraints_global, required_ujss):
         """constraints_global: [[x_c, *values_c], ...] (CPU or GPU tensors)."""
         self.cglobal = [[tree_to(c_, self.device) for c_ in c] for c in constraints_global]
+        if self.sort_points and self.problem.residuals(self.all_params) is not None:
+            # a loss stated as means of pointwise residuals does not depend on
+            # the order of a constraint's points (up to the summation order):
+            # order them along a space-filling curve, for locality in the
+            # kernels' gathers
+            for c in self.cglobal:
+                if not all(isinstance(v, torch.Tensor) and v.shape[:1] == c[0].shape[:1] for v in c):
+                    raise FbpxError("constraint values must be per-point tensors")
+                perm = torch.as_tensor(spatial_order(c[0].detach().cpu().numpy()), dtype=torch.long,
+                                       device=self.device)
+                c[:] = [v[perm].contiguous() for v in c]
         self.required_ujss = [tuple((iu, tuple(ixs)) for iu, ixs in r) for r in required_ujss]
         self.x_global = torch.cat([c[0] for c in self.cglobal]).to(torch.float32).contiguous()
         sizes = [c[0].shape[0] for c in self.cglobal]

@@ -136,6 +136,29 @@ def split_takes(takes, constraint_fs, constraint_ips, n_train):
     return takess
 
 
+def spatial_order(x):
+    """Permutation of the rows of x (n, xd) along a Morton (Z-order) curve over
+    their bounding box (21 bits per axis for xd = 1 to 3, ties in input order).
+
+    In this order the points of one subdomain form a few contiguous runs, so
+    the pair kernels' gathers of points and of upstream jets, and the forward's
+    scattered writes of pair jets, share cache lines instead of touching one
+    line per pair."""
+    x = np.asarray(x, dtype=np.float64)
+    n, xd = x.shape
+    if n < 2:
+        return np.arange(n)
+    bits = min(21, 63 // max(xd, 1))
+    lo, hi = x.min(0), x.max(0)
+    span = np.where(hi > lo, hi - lo, 1.0)
+    q = np.floor((x - lo) / span * ((1 << bits) - 1)).astype(np.uint64)
+    code = np.zeros(n, dtype=np.uint64)
+    for b in range(bits):
+        for i in range(xd):
+            code |= ((q[:, i] >> np.uint64(b)) & np.uint64(1)) << np.uint64(b * xd + i)
+    return np.argsort(code, kind="stable")
+
+
 class WorkPlan:
     """Work list plus the row bookkeeping of one set of takes."""
 

@@ -123,7 +123,7 @@ class FBPINNTrainer(_Trainer):
         c = self.c
         return FusedEngine(c.problem, c.decomposition, c.network, all_params, c.optimiser_kwargs,
                            device=c.device, graph=c.graph,
-                           autotune=c.autotune, fused_loss=c.fused_loss)
+                           autotune=c.autotune, fused_loss=c.fused_loss, sort_points=c.sort_points)
 
     def setup(self, exact=True):
         """Parameters, constraints, test points and the engine (no steps)."""
@@ -219,7 +219,7 @@ class PINNTrainer(FBPINNTrainer):
         c = self.c
         return FusedEngine(c.problem, RectangularDecompositionND, c.network, all_params, c.optimiser_kwargs,
                            device=c.device, graph=c.graph,
-                           autotune=c.autotune, fused_loss=c.fused_loss)
+                           autotune=c.autotune, fused_loss=c.fused_loss, sort_points=c.sort_points)
 
     def train(self):
         from fbpinns_amd.schedulers import AllActiveSchedulerND

@@ -155,6 +155,41 @@ def test_build_worklist_bins():
     assert np.array_equal(plan.wl.t["bbin_chunk_start"].numpy(), np.arange(n_chunks + 1))
 
 
+def test_spatial_order():
+    """Z-order permutation of training points: 1-D is the stable sort (ties
+    keep input order); a 4 x 4 grid given in shuffled order comes out in the
+    Z-curve order listed by hand (axis 0 the low bit of each level); for
+    uniform 3-D points each subdomain's points form a few long runs instead of
+    one run per point."""
+    r = np.random.default_rng(3)
+    x1 = r.uniform(0, 1, (500, 1))
+    x1[100:110] = x1[5]
+    assert np.array_equal(ptakes.spatial_order(x1), np.argsort(x1[:, 0], kind="stable"))
+    z = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (3, 0), (2, 1), (3, 1),
+         (0, 2), (1, 2), (0, 3), (1, 3), (2, 2), (3, 2), (2, 3), (3, 3)]
+    shuffle = r.permutation(16)
+    g = np.array(z, dtype=np.float64)[shuffle] * 0.25 + 0.1
+    out = [tuple(int(round(v)) for v in (g[i] - 0.1) / 0.25) for i in ptakes.spatial_order(g)]
+    assert out == z
+    assert ptakes.spatial_order(np.zeros((1, 3))).tolist() == [0]
+    assert ptakes.spatial_order(np.zeros((0, 2))).tolist() == []
+    x = r.uniform(0, 1, (20000, 3))
+    order = ptakes.spatial_order(x)
+    assert np.array_equal(np.sort(order), np.arange(len(x)))
+    _, dec = setups.decomposition_params((6, 6, 3), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1.9)
+    lo, hi = dec["subdomain"]["params"][0].numpy(), dec["subdomain"]["params"][1].numpy()
+
+    def runs_per_point(o):
+        xs = x[o]
+        n_runs = n_pts = 0
+        for j in range(lo.shape[0]):
+            idx = np.nonzero(np.all((xs >= lo[j]) & (xs <= hi[j]), 1))[0]
+            n_runs += 1 + int((np.diff(idx) > 1).sum())
+            n_pts += len(idx)
+        return n_runs / n_pts
+    assert runs_per_point(order) < 0.2 < 0.8 < runs_per_point(np.arange(len(x)))
+
+
 # ------------------------------------------------------------------ epilogue
 
 def test_get_jmaps_matches_oracle():
