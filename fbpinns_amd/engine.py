@@ -75,7 +75,13 @@ def network_shape(layers):
 
 
 class FusedEngine:
-    """Owns the packed device state and runs train steps for one problem."""
+    """Owns the packed device state and runs train steps for one problem.
+
+    graph: capture the whole step in a hipGraph; autotune: time candidate
+    launch shapes per work list; fused_loss: run a problem's residual
+    description through the fused residual kernels (else its loss_fn);
+    sort_points: keep each constraint's points in Z-order when the problem
+    has a residual description (its loss is then order-independent)."""
 
     def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
                  graph=True, autotune=True, fused_loss=True, sort_points=True):
