@@ -3,6 +3,7 @@ graph-captured step on one MI355X, against the plain per-op GPU form of the
 same method and a bounded CPU sample of the oracle restatement.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config wave]
+python bench.py --config C --train-l1     (full-length training, test L1 history)
 
 Prints ONE JSON line.  Default workload: the (2+1)D wave configuration
 (1,048,576 collocation points, 12x12x6 subdomains, overlap 1.9, [3,64,64,1]).
