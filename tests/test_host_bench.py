@@ -1,0 +1,41 @@
+"""CPU: bench.py's contract pieces that need no GPU -- more than one GPU is
+refused with a message, and the calibrated HBM traffic it reports is the
+figure recorded in the committed rocprofv3 summary."""
+
+import os
+import re
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr or "one MI355X" in r.stderr
+    assert r.stdout == ""
+
+
+def test_traffic_figure_comes_from_the_profile_summary():
+    import bench
+    with open(os.path.join(ROOT, "profiles", "wave_traffic.md")) as f:
+        text = f.read()
+    rec = {m.group(1): int(m.group(2))
+           for m in re.finditer(r"^(\w+) traffic_bytes_per_launch = (\d+)$", text, re.M)}
+    assert set(rec) == {"bwd_kernel", "fwd_kernel"}
+    assert bench.measured_traffic("wave", "bwd_kernel") == rec["bwd_kernel"]
+    assert bench.measured_traffic("wave", "fwd_kernel") == rec["fwd_kernel"]
+    assert bench.measured_traffic("poisson", "bwd_kernel") is None
+    # the recorded bytes are the table's counters scaled by the copy calibration
+    rows = {m.group(1): (float(m.group(2)), float(m.group(3)))
+            for m in re.finditer(r"^\| (\w+)<[^|]*\| ([\d.]+) \| ([\d.]+) \|", text, re.M)}
+    copy = re.search(r"^\| copy \(1 GiB in, 1 GiB out\) \| ([\d.]+) \| ([\d.]+) \|", text, re.M)
+    f_fetch = 1024 * 1024 / float(copy.group(1))      # KiB fetched for 1 GiB read
+    f_write = 1024 * 1024 / float(copy.group(2))
+    assert abs(f_fetch - 1.99995) < 1e-4 and f_write == 1.0
+    for k in ("bwd_kernel", "fwd_kernel"):
+        fetch, write = rows[k]
+        assert abs(round(fetch * 1.99995 * 1024 + write * f_write * 1024) - rec[k]) <= 1
