@@ -13,10 +13,12 @@ a contiguous run of chunks (see fbpx_worklist).  Two families of launch shapes:
 
 tune_plan() builds the work list for every candidate, times one forward and
 one backward launch on each (HIP events on the current stream, after a warm-up
-launch) and returns the work list of the faster family: the best chunk size
-(forward + backward), or persistent bins with each kernel's best multiplier.
-The families are not mixed: another kernel's bin edges would split the chunks
-of a per-chunk launch into fragments.  Results are cached per (network and jet
+launch) and returns the work list with the least forward + backward time
+among: the best chunk size for both kernels, persistent bins with each
+kernel's best multiplier, and the two mixed forms (one kernel on its best
+bins, the other on its best chunk size).  A mixed form is timed on its own
+work list, since one kernel's bin edges split the other's chunks and a chunk
+cap splits the bins' chunks.  Results are cached per (network and jet
 shape, subdomain count, pair count to two significant digits), so a scheduler
 that revisits a shape does not tune again.  Work lists below MIN_PAIRS take
 DEFAULT_SMALL without timing: their step is launch-bound and the timings would
@@ -79,7 +81,7 @@ def name(spec):
 
 def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
     """build(fspec, bspec) -> WorkPlan for the current takes.  Returns
-    ((fspec, bspec), WorkPlan) of the fastest launch family."""
+    ((fspec, bspec), WorkPlan) of the fastest launch pair."""
     if n_pairs < MIN_PAIRS:
         spec = DEFAULT_SMALL
         return (spec, spec), build(spec, spec)
@@ -90,12 +92,27 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
     tf, tb = {}, {}
     for spec in CANDIDATES:
         tf[spec], tb[spec] = _time_launches(net, theta, dom, x, build(spec, spec), reps)
+    (fs, bs), cand = select(tf, tb, lambda f, b: _time_launches(net, theta, dom, x, build(f, b), reps))
     torch.cuda.synchronize()
-    cs = min(CHUNKS, key=lambda c: tf[c] + tb[c])
-    fbin, bbin = min(BINS, key=lambda c: tf[c]), min(BINS, key=lambda c: tb[c])
-    fs, bs = (fbin, bbin) if tf[fbin] + tb[bbin] < tf[cs] + tb[cs] else (cs, cs)
     _cache[key] = (fs, bs)
+    mixed = [k for k in cand if k[0] != k[1] and {k[0][0], k[1][0]} == {"chunks", "bins"}]
     logger.info("autotune launch shape: " + ", ".join(
         f"{name(c)}: fwd {tf[c]:.3f} ms bwd {tb[c]:.3f} ms" for c in CANDIDATES)
+        + "".join(f", fwd {name(f)} + bwd {name(b)}: {cand[(f, b)][0]:.3f} + {cand[(f, b)][1]:.3f} ms"
+                  for f, b in mixed)
         + f" -> fwd {name(fs)}, bwd {name(bs)}")
     return (fs, bs), build(fs, bs)
+
+
+def select(tf, tb, measure):
+    """Launch pair (forward spec, backward spec) with the least forward +
+    backward time.  tf, tb: per-candidate times measured with both kernels
+    on that candidate; measure(fspec, bspec) -> (forward ms, backward ms) on
+    the mixed work list.  Returns (pair, {pair: (forward ms, backward ms)})."""
+    cs = min(CHUNKS, key=lambda c: tf[c] + tb[c])
+    cf, cb = min(CHUNKS, key=lambda c: tf[c]), min(CHUNKS, key=lambda c: tb[c])
+    fbin, bbin = min(BINS, key=lambda c: tf[c]), min(BINS, key=lambda c: tb[c])
+    cand = {(cs, cs): (tf[cs], tb[cs]), (fbin, bbin): (tf[fbin], tb[bbin])}
+    for pair in ((fbin, cb), (cf, bbin)):
+        cand[pair] = tuple(measure(*pair))
+    return min(cand, key=lambda k: sum(cand[k])), cand

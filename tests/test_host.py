@@ -190,6 +190,35 @@ def test_spatial_order():
     assert runs_per_point(order) < 0.2 < 0.8 < runs_per_point(np.arange(len(x)))
 
 
+def test_autotune_select():
+    """The autotuner's choice: the least forward + backward time among one
+    chunk size for both kernels, per-kernel best bins, and the two mixed forms
+    timed on their own work lists (a mixed pair is measured, not assembled
+    from the uniform timings)."""
+    from fbpinns_amd import autotune as A
+    c256, c512, b1, b2 = ("chunks", 256), ("chunks", 512), ("bins", 1), ("bins", 2)
+    # the poisson-like case: forward fastest on bins, backward on 256-pair chunks
+    tf = {c256: 0.164, c512: 0.122, b1: 0.120, b2: 0.124}
+    tb = {c256: 0.313, c512: 0.335, b1: 0.331, b2: 0.352}
+    asked = []
+
+    def measure(f, b):
+        asked.append((f, b))
+        return {(b1, c256): (0.121, 0.314), (c512, b1): (0.123, 0.333)}[(f, b)]
+    pair, cand = A.select(tf, tb, measure)
+    assert sorted(asked) == sorted([(b1, c256), (c512, b1)])
+    assert pair == (b1, c256) and cand[pair] == (0.121, 0.314)
+    assert cand[(c512, c512)] == (0.122, 0.335) and cand[(b1, b1)] == (0.120, 0.331)
+    # a mixed form that measures slower than its parts loses
+    pair, _ = A.select(tf, tb, lambda f, b: (0.2, 0.4))
+    assert pair == (b1, b1)
+    # the wave-like case: bins for both
+    tf = {c256: 2.95, c512: 2.94, b1: 2.90, b2: 2.89}
+    tb = {c256: 7.88, c512: 7.81, b1: 7.65, b2: 7.70}
+    pair, _ = A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05))
+    assert pair == (b2, b1)
+
+
 # ------------------------------------------------------------------ epilogue
 
 def test_get_jmaps_matches_oracle():
