@@ -66,7 +66,8 @@ CASES = {
 }
 
 
-def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True):
+def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True, dec_hi=None):
+    """dec_hi: upper corner of the decomposition's box (default: the domain's)."""
     prob, kw, lo, hi, nsub, ov, ls, ns, sampler, ofac, has_mu = CASES[name]
     if ov is None:   # cfg 1: 15 centres, width 0.15
         from fbpinns_amd.decompositions import RectangularDecompositionND
@@ -79,7 +80,7 @@ def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True):
         net = _stack_subdomain_params([FCN.init_params(k, ls) for k in sub])
         ap = {"static": {"decomposition": dec}, "trainable": {"network": {"subdomain": net}}}
     else:
-        dec_cls, ap = setups.build(nsub, lo, hi, ov, ls, seed=seed)
+        dec_cls, ap = setups.build(nsub, lo, hi if dec_hi is None else dec_hi, ov, ls, seed=seed)
     ap["static"]["domain"] = RectangularDomainND.init_params(np.array(lo), np.array(hi))[0]
     ps = prob.init_params(**kw)
     ap["static"]["problem"] = ps[0]
@@ -133,6 +134,17 @@ def test_loss_and_gradients(name):
     el = abs(loss.item() - oloss.item()) / abs(oloss.item())
     print(f"  loss gpu {loss.item():.8e} oracle {oloss.item():.8e} rel {el:.2e}")
     assert el <= TOL
+    _check_layer_grads(g, ogl, all_ims)
+    if has_mu:
+        e = abs(gp[0].item() - ogx[0].item()) / abs(ogx[0].item())
+        print(f"  dmu gpu {gp[0].item():.6e} oracle {ogx[0].item():.6e} rel {e:.2e}")
+        assert e <= TOL
+    assert (eng.plan.wl.n_points == sum(c[0].shape[0] for c in constraints))
+
+
+def _check_layer_grads(g, ogl, all_ims):
+    """Per layer tensor: max |gpu - oracle| <= TOL * max |oracle| over the
+    used subdomains' rows of the packed gradient g [m][P]."""
     g = g.detach().cpu()
     off = 0
     for (gW, gb) in ogl:
@@ -145,11 +157,32 @@ def test_loss_and_gradients(name):
             e = ((a.double() - b).abs().max() / b.abs().max().clamp_min(1e-30)).item()
             print(f"  d{nm}{tuple(b.shape)} rel {e:.2e}")
             assert e <= TOL, (nm, e)
-    if has_mu:
-        e = abs(gp[0].item() - ogx[0].item()) / abs(ogx[0].item())
-        print(f"  dmu gpu {gp[0].item():.6e} oracle {ogx[0].item():.6e} rel {e:.2e}")
-        assert e <= TOL
-    assert (eng.plan.wl.n_points == sum(c[0].shape[0] for c in constraints))
+
+
+def test_step_with_subdomains_without_points():
+    """Part of the decomposition holds no training points (subdomains over
+    [0, 2] x [0, 1], points in [0, 1]^2): loss and gradients match the
+    oracle, the empty subdomains' gradient rows are zero, and three steps
+    leave their parameters unchanged and their Adam moments zero."""
+    eng, ap, constraints, ofac, has_mu = _build("poisson_small", dec_hi=(2.0, 1.0))
+    loss, g, _ = eng.loss_and_grad()
+    torch.cuda.synchronize()
+    oloss, ogl, _, all_ims = _oracle_loss_grad(eng, constraints, ofac, False)
+    print(f"[empty] m={eng.m} used={len(all_ims)} loss gpu {loss.item():.8e} oracle {oloss.item():.8e}",
+          flush=True)
+    assert 0 < len(all_ims) < eng.m
+    assert abs(loss.item() - oloss.item()) <= TOL * abs(oloss.item())
+    _check_layer_grads(g, ogl, all_ims)
+    empty = torch.as_tensor(np.setdiff1d(np.arange(eng.m), all_ims))
+    assert (g.detach().cpu()[empty] == 0).all()
+    th0 = eng.theta.detach().cpu().clone()
+    for _ in range(3):
+        eng.step()
+    torch.cuda.synchronize()
+    assert torch.equal(eng.theta.detach().cpu()[empty], th0[empty])
+    assert (eng.mu.detach().cpu()[empty] == 0).all() and (eng.nu.detach().cpu()[empty] == 0).all()
+    used = torch.as_tensor(all_ims)
+    assert (eng.theta.detach().cpu()[used] != th0[used]).any(1).all()
 
 
 def test_adam_matches_oracle():
