@@ -226,6 +226,8 @@ def main():
                     help="keep the default launch shape (counter passes see one launch shape)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--overlap", type=float, default=None,
+                    help="wave/poisson/burgers: subdomain overlap factor in place of the configuration's")
     ap.add_argument("--train-l1", action="store_true",
                     help="only train --config for its full step count and print its test L1 history")
     a = ap.parse_args()
@@ -247,7 +249,15 @@ def main():
     for h in logger.handlers:
         h.setStream(sys.stderr)   # stdout carries only the JSON line
 
-    c = configs.CONFIGS[a.config](graph=not a.no_graph, autotune=not a.no_autotune)
+    over = {}
+    if a.overlap is not None:
+        if a.config not in ("wave", "poisson", "burgers"):
+            print(f"--overlap applies to the wave, poisson and burgers configurations", file=sys.stderr)
+            sys.exit(2)
+        xs = configs.CONFIGS[a.config]().decomposition_init_kwargs["subdomain_xs"]
+        lo, hi = [v[0] for v in xs], [v[-1] for v in xs]
+        over = dict(decomposition_init_kwargs=configs._rect(lo, hi, [len(v) for v in xs], a.overlap))
+    c = configs.CONFIGS[a.config](graph=not a.no_graph, autotune=not a.no_autotune, **over)
     log(f"config {a.config}: building problem")
     tr = FBPINNTrainer(c, write_outputs=False)
     eng, sched, x_test, _ = tr.setup(exact=False)
@@ -276,12 +286,14 @@ def main():
     dom_is_bwd = rl["t_bwd_ms"] >= rl["t_fwd_ms"]
     ach = rl["tflops_bwd"] if dom_is_bwd else rl["tflops_fwd"]
     dom_kernel = "bwd_kernel" if dom_is_bwd else "fwd_kernel"
-    traffic = measured_traffic(a.config, dom_kernel)
+    # the recorded counters are for the configuration's own overlap
+    traffic = measured_traffic(a.config, dom_kernel) if not over else None
 
     plain = None
     if not a.no_plain:
         log("plain per-op form")
-        eng2 = FBPINNTrainer(configs.CONFIGS[a.config](graph=False), write_outputs=False).setup(exact=False)[0]
+        eng2 = FBPINNTrainer(configs.CONFIGS[a.config](graph=False, **over),
+                             write_outputs=False).setup(exact=False)[0]
         pl = PlainFBPINN(c.problem, c.decomposition, c.network, eng2,
                          point_chunks=a.plain_chunks if len(eng2.cglobal) == 1 else 1)
         pl.set_active(active)
@@ -328,7 +340,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded collocation points, random-init subdomain networks)",
-        "config": {"workload": f"{a.config}: {n_pts} collocation points, {eng.m} subdomains, "
+        "config": {"workload": f"{a.config}{'' if not over else f' (overlap {a.overlap})'}: "
+                               f"{n_pts} collocation points, {eng.m} subdomains, "
                                f"{eng.plan.wl.n_pairs} (point, subdomain) pairs, FCN [{eng.net.xd}, "
                                f"{eng.net.h}x{eng.net.nh}, 1], jets J={eng.net.J}, hipGraph-captured step",
                    "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs,
