@@ -2,7 +2,6 @@
 returned as float32 torch tensors on the CPU (the trainer moves them)."""
 
 import numpy as np
-import scipy.stats
 import torch
 
 from fbpinns_amd import networks
@@ -46,28 +45,6 @@ class RectangularDomainND(Domain):
         return RectangularDomainND._rectangle_samplerND(key, sampler, d["xmin"], d["xmax"], batch_shape)
 
     @staticmethod
-    def sample_boundaries(all_params, key, sampler, batch_shapes):
-        """domains.py:81-103: one batch per face, in order (x0min, x0max, x1min, ...)."""
-        d = all_params["static"]["domain"]
-        xmin, xmax, xd = d["xmin"], d["xmax"], d["xd"]
-        assert len(batch_shapes) == 2 * xd
-        out = []
-        for i in range(xd):
-            ic = [j for j in range(xd) if j != i]
-            for j, v in enumerate([xmin[i], xmax[i]]):
-                bs = batch_shapes[2 * i + j]
-                if ic:
-                    key, sub = rng.split(key) if key is not None else (None, None)
-                    xb_ = RectangularDomainND._rectangle_samplerND(sub, sampler, xmin[ic], xmax[ic], bs)
-                    xb = torch.full((int(np.prod(bs)), xd), float(v), dtype=torch.float32)
-                    xb[:, ic] = xb_
-                else:
-                    assert len(bs) == 1
-                    xb = torch.full(tuple(bs) + (1,), float(v), dtype=torch.float32)
-                out.append(xb)
-        return out
-
-    @staticmethod
     def norm_fn(all_params, x):
         d = all_params["static"]["domain"]
         xmin = torch.as_tensor(d["xmin"], dtype=x.dtype, device=x.device)
@@ -76,26 +53,23 @@ class RectangularDomainND(Domain):
 
     @staticmethod
     def _rectangle_samplerND(key, sampler, xmin, xmax, batch_shape):
-        """domains.py:112-142.  grid: per-axis linspace (computed in float64,
-        rounded once to float32) on an ij meshgrid; uniform / sobol / halton:
-        xmin + (xmax - xmin) * s."""
+        """domains.py:112-142 for the samplers the configurations use.  grid:
+        per-axis linspace (computed in float64, rounded once to float32) on an
+        ij meshgrid; uniform: xmin + (xmax - xmin) * s, s from the seeded key.
+        The reference's quasi-random samplers (sobol, halton) and boundary
+        sampling are not part of this engine and raise."""
         xmin, xmax = np.asarray(xmin, dtype=np.float64), np.asarray(xmax, dtype=np.float64)
         assert xmin.shape == xmax.shape and xmin.ndim == 1
         xd = len(xmin)
         assert len(batch_shape) == xd
-        if sampler not in ["grid", "uniform", "sobol", "halton"]:
-            raise ValueError("ERROR: unexpected sampler")
+        if sampler not in ("grid", "uniform"):
+            raise ValueError(f"sampler {sampler!r} is not supported (grid, uniform)")
         if sampler == "grid":
             xs = [np.linspace(a, b, n) for a, b, n in zip(xmin, xmax, batch_shape)]
             xx = np.stack(np.meshgrid(*xs, indexing="ij"), -1).reshape((-1, xd))
             return torch.tensor(xx, dtype=torch.float32)
         n = int(np.prod(batch_shape))
-        if sampler == "halton":
-            s = scipy.stats.qmc.Halton(xd).random(n)
-        elif sampler == "sobol":
-            s = scipy.stats.qmc.Sobol(xd).random(n)
-        else:
-            s = rng.uniform(key, (n, xd))
+        s = rng.uniform(key, (n, xd))
         x = xmin[None] + (xmax - xmin)[None] * s
         return torch.tensor(x, dtype=torch.float32)
 

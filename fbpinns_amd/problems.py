@@ -292,22 +292,6 @@ class WaveEquationConstantVelocity3D(Problem):
         return torch.tensor(wavefields.reshape((-1, 1)), dtype=torch.float32)
 
 
-class WaveEquationGaussianVelocity3D(WaveEquationConstantVelocity3D):
-    """Gaussian-mixture velocity model (problems.py:484-522)."""
-
-    @staticmethod
-    def init_params(c0=1, source=np.array([[0.0, 0.0, 0.2, 1.0]]), mixture=np.array([[0.5, 0.5, 1.0, 0.2]])):
-        return ({"dims": (1, 3), "c0": c0, "c_fn": WaveEquationGaussianVelocity3D.c_fn,
-                 "source": torch.tensor(np.asarray(source), dtype=torch.float32),
-                 "mixture": torch.tensor(np.asarray(mixture), dtype=torch.float32)}, {})
-
-    @staticmethod
-    def c_fn(all_params, x_batch):
-        c0 = all_params["static"]["problem"]["c0"]
-        mixture = _t(all_params["static"]["problem"]["mixture"], x_batch)
-        return c0 + _gaussian_sum(mixture, x_batch[:, 0:2])
-
-
 class PoissonMultiscale2D(Problem):
     """u_xx + u_yy = sum_{k in ks} sin(2^k pi x) sin(2^k pi y) on [0,1]^2 with
     zero Dirichlet BCs through the hard ansatz

@@ -32,97 +32,15 @@ class AllActiveSchedulerND(ActiveScheduler):
             yield np.ones(self.m, dtype=int) if i == 0 else None
 
 
-class _SubspacePointSchedulerRectangularND(ActiveScheduler):
-    """Activates subdomains as a radius grows from a point in a subspace
-    (schedulers.py:52-135)."""
-
-    def __init__(self, all_params, n_steps, point, iaxes):
-        super().__init__(all_params, n_steps)
-        point = np.array(point)
-        iaxes = list(iaxes)
-        if point.ndim != 1:
-            raise Exception("ERROR: point.ndim != 1")
-        if len(point) > self.xd:
-            raise Exception("ERROR: len(point) > self.xd")
-        if len(iaxes) + len(point) != self.xd:
-            raise Exception("ERROR: len(iaxes) + len(point) != self.xd")
-        self.point, self.iaxes = point, iaxes
-        self.xmins0 = np.array(all_params["static"]["decomposition"]["xmins0"]).copy()
-        self.xmaxs0 = np.array(all_params["static"]["decomposition"]["xmaxs0"]).copy()
-
-    @staticmethod
-    def _get_radii(point, xmins, xmaxs):
-        """Shortest and longest distance from point to each box (schedulers.py:74-103)."""
-        assert xmins.shape[1] == xmaxs.shape[1] == point.shape[0]
-        point = point[None]
-        c_inside = np.prod((point >= xmins) & (point <= xmaxs), axis=1).astype(bool)
-        pmin = np.clip(point, xmins, xmaxs)
-        ds = np.stack([point - xmins, point - xmaxs], axis=0)
-        i = np.argmax(np.abs(ds), axis=0)[None]
-        pmax = point - np.take_along_axis(ds, i, axis=0)[0]
-        rmin = np.sqrt(np.sum((pmin - point) ** 2, axis=1))
-        rmax = np.sqrt(np.sum((pmax - point) ** 2, axis=1))
-        rmin[c_inside] = 0.0
-        return rmin, rmax
-
-    def __iter__(self):
-        ic = [i for i in range(self.xd) if i not in self.iaxes]
-        rmin, rmax = self._get_radii(self.point, self.xmins0[:, ic], self.xmaxs0[:, ic])
-        r_min, r_max = rmin.min(), rmax.max()
-        active = np.zeros(self.m, dtype=int)
-        for i in range(self.n_steps):
-            rt = r_min + (r_max - r_min) * (i / self.n_steps)
-            c_radius = (rt >= rmin) & (rt < rmax)
-            c_to_active = (active == 0) & c_radius
-            c_to_fixed = (active == 1) & (~c_radius)
-            if c_to_active.any() or c_to_fixed.any():
-                active[c_to_active] = 1
-                active[c_to_fixed] = 2
-                yield active.copy()
-            else:
-                yield None
-
-
-class PointSchedulerRectangularND(_SubspacePointSchedulerRectangularND):
-    """schedulers.py:137-143."""
-
-    def __init__(self, all_params, n_steps, point):
-        xd = all_params["static"]["decomposition"]["xd"]
-        if len(point) != xd:
-            raise Exception(f"ERROR: point incorrect shape {np.shape(point)}")
-        super().__init__(all_params, n_steps, point, iaxes=[])
-
-
-class LineSchedulerRectangularND(_SubspacePointSchedulerRectangularND):
-    """schedulers.py:145-152."""
-
-    def __init__(self, all_params, n_steps, point, iaxis):
-        xd = all_params["static"]["decomposition"]["xd"]
-        if xd < 2:
-            raise Exception("ERROR: requires nd >=2")
-        if len(point) != xd - 1:
-            raise Exception(f"ERROR: point incorrect shape {np.shape(point)}")
-        super().__init__(all_params, n_steps, point, iaxes=[iaxis])
-
-
-class PlaneSchedulerRectangularND(_SubspacePointSchedulerRectangularND):
-    """schedulers.py:154-161."""
-
-    def __init__(self, all_params, n_steps, point, iaxes):
-        xd = all_params["static"]["decomposition"]["xd"]
-        if xd < 3:
-            raise Exception("ERROR: requires nd >=3")
-        if len(point) != xd - 2:
-            raise Exception(f"ERROR: point incorrect shape {np.shape(point)}")
-        super().__init__(all_params, n_steps, point, iaxes=iaxes)
-
-
 class SlabSchedulerRectangularND(ActiveScheduler):
     """Time-marching slabs: the subdomain grid is cut along `axis` into slabs
     of `per_slab` subdomain layers (default 1); slab k is active for
     `steps_per_slab` steps, slabs < k are fixed, slabs > k inactive.  Follows
-    the Line scheduler's state semantics (schedulers.py:117-135) with a fixed
-    step count per slab."""
+    the reference schedulers' state semantics (schedulers.py:117-135: 0
+    inactive, 1 active, 2 fixed) with a fixed step count per slab.  The
+    reference's point / line / plane schedulers (schedulers.py:52-161) are not
+    part of this engine; a user scheduler yielding the same states drives it
+    the same way."""
 
     def __init__(self, all_params, n_steps, axis, steps_per_slab, per_slab=1):
         super().__init__(all_params, n_steps)

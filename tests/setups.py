@@ -6,6 +6,7 @@ import torch
 from fbpinns_amd.constants import get_subdomain_ws
 from fbpinns_amd.decompositions import MultilevelRectangularDecompositionND, RectangularDecompositionND
 from fbpinns_amd.networks import FCN
+from fbpinns_amd.problems import WaveEquationConstantVelocity3D
 from fbpinns_amd.trainers import _stack_subdomain_params
 from fbpinns_amd.util import rng
 
@@ -91,3 +92,26 @@ def col_rel_err(a, b):
     num = (a - b).abs().amax(dim=1)
     den = b.abs().amax(dim=1).clamp_min(1e-30)
     return (num / den)
+
+
+
+class VariableVelocityWave3D(WaveEquationConstantVelocity3D):
+    """Test problem: the (2+1)D wave problem with a position-dependent
+    velocity c(x, y) = c0 + a sum of Gaussian bumps, so that the residual's
+    -1/c^2 factor varies per point (exercises per-point static factors of the
+    fused residual loss)."""
+
+    @staticmethod
+    def init_params(c0=1, source=np.array([[0.0, 0.0, 0.2, 1.0]]), mixture=np.array([[0.5, 0.5, 1.0, 0.2]])):
+        return ({"dims": (1, 3), "c0": c0, "c_fn": VariableVelocityWave3D.c_fn,
+                 "source": torch.tensor(np.asarray(source), dtype=torch.float32),
+                 "mixture": torch.tensor(np.asarray(mixture), dtype=torch.float32)}, {})
+
+    @staticmethod
+    def c_fn(all_params, x_batch):
+        c0 = all_params["static"]["problem"]["c0"]
+        p = torch.as_tensor(all_params["static"]["problem"]["mixture"], dtype=x_batch.dtype,
+                            device=x_batch.device)[:, None, :]
+        xy = x_batch[:, 0:2]
+        return c0 + (p[:, :, 3:4] * torch.exp(-0.5 * ((xy[None] - p[:, :, 0:2]) ** 2).sum(2, keepdim=True)
+                                               / (p[:, :, 2:3] ** 2))).sum(0)

@@ -373,61 +373,6 @@ def test_schedulers():
     a0, a1 = sl[0].reshape(5, 4), sl[2].reshape(5, 4)
     assert np.all(a0[:, 0] == 1) and np.all(a0[:, 1:] == 0)
     assert np.all(a1[:, 0] == 2) and np.all(a1[:, 1] == 1) and np.all(a1[:, 2:] == 0)
-    ln = S.LineSchedulerRectangularND(ap, 100, point=[0.0], iaxis=0)
-    states = [a for a in ln if a is not None]
-    assert len(states) >= 2 and np.all(np.isin(states[-1], [1, 2]))
-    rmin, rmax = S._SubspacePointSchedulerRectangularND._get_radii(np.array([0.5]), np.array([[0.0], [1.0]]),
-                                                                   np.array([[1.0], [2.0]]))
-    assert np.allclose(rmin, [0.0, 0.5]) and np.allclose(rmax, [0.5, 1.5])
-
-
-def _grid_dec(nsub, lo, hi, overlap=2.0):
-    xs = [np.linspace(a, b, n) for a, b, n in zip(lo, hi, nsub)]
-    return {"static": {"decomposition": RectangularDecompositionND.init_params(
-        xs, get_subdomain_ws(xs, overlap), (0, 1))[0]}}
-
-
-def _check_growth(states, m):
-    """Each subdomain goes 0 -> 1 -> 2 at most once each, never back; every
-    state changes something; in the end every subdomain has been active."""
-    assert states and all(s.shape == (m,) for s in states)
-    prev = np.zeros(m, dtype=int)
-    for s in states:
-        assert np.all(s >= prev) and np.all(np.isin(s, [0, 1, 2])) and np.any(s != prev)
-        prev = s
-    assert np.all(prev >= 1)
-
-
-def test_point_and_plane_schedulers():
-    """schedulers.py:137-161: the active set grows from a point (or a plane
-    through it) as the radius sweeps from the nearest to the farthest box;
-    exactly the boxes holding the point (plane) start active, the farthest box
-    does not; wrong point shapes raise."""
-    ap = _grid_dec((4, 3), (0.0, 0.0), (1.0, 1.0))
-    m = ap["static"]["decomposition"]["m"]
-    sch = S.PointSchedulerRectangularND(ap, 60, point=[0.1, 0.1])
-    states = [a for a in sch if a is not None]
-    _check_growth(states, m)
-    lo = np.array(ap["static"]["decomposition"]["xmins0"])
-    hi = np.array(ap["static"]["decomposition"]["xmaxs0"])
-    holds = np.all((lo <= 0.1) & (hi >= 0.1), axis=1)
-    assert np.array_equal(states[0] == 1, holds)
-    far = np.argmax(np.linalg.norm((lo + hi) / 2 - 0.1, axis=1))
-    assert states[0][far] == 0
-    ap3 = _grid_dec((3, 3, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
-    m3 = ap3["static"]["decomposition"]["m"]
-    pl = S.PlaneSchedulerRectangularND(ap3, 40, point=[0.0], iaxes=[0, 1])
-    states = [a for a in pl if a is not None]
-    _check_growth(states, m3)
-    lo3 = np.array(ap3["static"]["decomposition"]["xmins0"])
-    # a plane t = 0 starts with every subdomain whose t-range holds 0, whatever its x, y
-    assert np.array_equal(states[0] == 1, lo3[:, 2] <= 0.0)
-    with pytest.raises(Exception):
-        S.PointSchedulerRectangularND(ap, 10, point=[0.1])
-    with pytest.raises(Exception):
-        S.PlaneSchedulerRectangularND(ap, 10, point=[0.1], iaxes=[0])
-    with pytest.raises(Exception):
-        S.PlaneSchedulerRectangularND(ap3, 10, point=[0.1, 0.2], iaxes=[0])
 
 
 def test_constants_guard_and_domain_samplers():
@@ -440,5 +385,5 @@ def test_constants_guard_and_domain_samplers():
     assert np.allclose(g.numpy(), [[0, -1], [0, 1], [0.5, -1], [0.5, 1], [1, -1], [1, 1]])
     u = RectangularDomainND.sample_interior(ap, rng.PRNGKey(1), "uniform", (100, 1))
     assert u.shape == (100, 2) and (u[:, 0] >= 0).all() and (u[:, 1] >= -1).all() and (u[:, 1] <= 1).all()
-    bd = RectangularDomainND.sample_boundaries(ap, rng.PRNGKey(0), "grid", [(4,), (4,), (3,), (3,)])
-    assert torch.all(bd[0][:, 0] == 0) and torch.all(bd[1][:, 0] == 1) and torch.all(bd[3][:, 1] == 1)
+    with pytest.raises(ValueError):
+        RectangularDomainND.sample_interior(ap, None, "sobol", (3, 2))

@@ -12,6 +12,7 @@ import dataclasses
 import pytest
 import torch
 
+import setups
 from fbpinns_amd import epilogue
 from fbpinns_amd import problems as P
 from fbpinns_amd.engine import pick_schema
@@ -27,7 +28,7 @@ CASES = {
     "burgers": (P.BurgersEquation2D, dict(), 2, [((0, ()), (0, (0,)), (0, (1,)), (0, (0, 0)))], [0], [50]),
     "wave": (P.WaveEquationConstantVelocity3D, dict(c0=1.3, source=P.wave_sources()), 3,
              [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0], [50]),
-    "wave_gauss": (P.WaveEquationGaussianVelocity3D, dict(c0=1.0, source=P.wave_sources()), 3,
+    "wave_gauss": (setups.VariableVelocityWave3D, dict(c0=1.0, source=P.wave_sources()), 3,
                    [((0, (0, 0)), (0, (1, 1)), (0, (2, 2)))], [0], [50]),
     "poisson": (P.PoissonMultiscale2D, dict(), 2, [((0, (0, 0)), (0, (1, 1)))], [0], [50]),
     "batched_inverse": (P.BatchedHarmonicOscillatorInverse,
