@@ -126,7 +126,11 @@ __device__ inline void window_jets(const float* x, const Dom& d, float* w) {
     const float phi = __builtin_amdgcn_fmed3f((x[i] - mu) * (0.5f * k), -0.5f * kPi, 0.5f * kPi);
     float co, si;
     half_angle_cs(phi, co, si);
-    const float cc = co * co;
+    // 1 + cos(theta) = 2 c^2 rounds to 0 in the reference's float32 form once
+    // c^2 <= 2^-26: there the window and its derivatives are exactly 0, so a
+    // point on the edges of all its boxes has window sum 0 (u = 0/0 = NaN,
+    // trainers.py:166) as in the reference
+    const float cc = fabsf(co) <= 0x1p-13f ? 0.f : co * co;
     c[i] = cc * cc;
     c1[i] = (-2.f * k) * (si * (cc * co));
     c2[i] = (k * k) * (cc * fmaf(-4.f, cc, 3.f));

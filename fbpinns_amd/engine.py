@@ -185,6 +185,47 @@ class FusedEngine:
         tr["problem"] = prob
         return {"static": ap["static"], "trainable": tr}, leaf
 
+    # -------------------------------------------------------------- optimiser state
+
+    def _tree_of(self, rows, flat):
+        """A tree mirroring all_params["trainable"] whose leaves are views of
+        the packed rows [m][P] (network layers) and flat [Q] (problem leaves)."""
+        layers, off = [], 0
+        for W, b in self.all_params["trainable"]["network"]["subdomain"]["layers"]:
+            o, i = W.shape[1], W.shape[2]
+            layers.append((rows[:, off:off + o * i].view(self.m, o, i), rows[:, off + o * i:off + o * i + o]))
+            off += o * i + o
+        prob, off = {}, 0
+        for k, shp in zip(self.pkeys, self.pshapes):
+            n = int(np.prod(shp))
+            prob[k] = flat[off:off + n].view(shp)
+            off += n
+        tree = {}
+        for key in self.all_params["trainable"]:
+            if key == "network":
+                tree[key] = {"subdomain": {"layers": layers}}
+            elif key == "problem":
+                tree[key] = prob
+        return tree
+
+    def opt_state(self):
+        """optax.adam's state as (count, mu, nu): the shared int32 step count
+        and the first / second moments as trees mirroring
+        all_params["trainable"] (trainers.py:430-434)."""
+        return self.count.reshape(()), self._tree_of(self.mu, self.pmu), self._tree_of(self.nu, self.pnu)
+
+    def load_opt_state(self, count, mu, nu):
+        """Set the Adam state from (count, mu, nu) in the layout opt_state()
+        returns (e.g. read back from a checkpoint)."""
+        for dst_rows, dst_flat, tree in ((self.mu, self.pmu, mu), (self.nu, self.pnu, nu)):
+            view = self._tree_of(dst_rows, dst_flat)
+            for (W, b), (sW, sb) in zip(view["network"]["subdomain"]["layers"], tree["network"]["subdomain"]["layers"]):
+                W.copy_(torch.as_tensor(np.asarray(sW)))
+                b.copy_(torch.as_tensor(np.asarray(sb)))
+            for k in self.pkeys:
+                view["problem"][k].copy_(torch.as_tensor(np.asarray(tree["problem"][k]), dtype=torch.float32))
+        self.count.fill_(int(np.asarray(count)))
+
     # -------------------------------------------------------------- set-up
 
     def set_constraints(self, constraints_global, required_ujss):

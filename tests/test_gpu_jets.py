@@ -37,6 +37,9 @@ CASES = [
     ("3d_h32_value", (3, 2, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2.0, [3, 32, 32, 1], (0, 0),
      ("uniform", 2000), False),
     ("multilevel_1d", (5,), (0.0,), (1.0,), 2.3, [1, 32, 32, 1], (1, 1), ("uniform", 400), True),
+    # one subdomain covering the domain: window flag 0, the FBPINN is a PINN
+    # (decompositions.py:169-172, 199; PINNTrainer's model, trainers.py:780-943)
+    ("pinn_m1_flag0", (1, 1), (0.0, 0.0), (1.0, 1.0), 1.0, [2, 32, 32, 1], (3, 3), ("uniform", 1500), False),
 ]
 
 
@@ -181,3 +184,28 @@ def test_inside_empty_and_edges():
     assert n2.numel() == 0 and c2.numel() == 0
     n3, m3, c3 = ops.inside_takes(x, xmins[:0], xmaxs[:0])
     assert n3.numel() == 0 and c3.tolist() == [0] * 5
+
+
+def test_point_with_zero_window_sum_is_nan():
+    """Boxes [0, 0.6] and [0.4, 1] span [0, 1] exactly: the points 0 and 1 lie
+    on the outer edge of the only box holding them (inclusive test), whose
+    window is exactly 0 there, so the reference's u / wp (trainers.py:166) is
+    0/0 = NaN in every jet column; the other points match the oracle."""
+    from fbpinns_amd.decompositions import RectangularDecompositionND
+    from fbpinns_amd.networks import FCN
+    from fbpinns_amd.trainers import _stack_subdomain_params
+    from fbpinns_amd.util import rng
+    dec = RectangularDecompositionND.init_params([np.array([0.3, 0.7])], [np.array([0.6, 0.6])], (0.0, 1.0))[0]
+    _, *sub = rng.split(rng.PRNGKey(0), 3)
+    net = _stack_subdomain_params([FCN.init_params(k, [1, 16, 1]) for k in sub])
+    ap = {"static": {"decomposition": dec}, "trainable": {"network": {"subdomain": net}}}
+    x = torch.tensor([[0.0], [0.1], [0.5], [0.9], [1.0]], dtype=torch.float32)
+    eng = FusedEngine(Problem, RectangularDecompositionND, FCN, ap, device="cuda:0", graph=False)
+    plan, takes = eng.make_plan(x.cuda(), schema=(1, 1))
+    with torch.no_grad():
+        ujet = FBPINNJets.apply(eng.theta, plan).cpu()
+    ref, _, _ = setups.oracle_jets(ap, x, 1, 1, 1)
+    print(f"[wp=0] gpu {ujet.tolist()} oracle {ref.tolist()}", flush=True)
+    edge, ok = [0, 4], [1, 2, 3]
+    assert torch.isnan(ref[:, edge]).all() and torch.isnan(ujet[:, edge]).all()
+    assert setups.col_rel_err(ujet[:, ok], ref[:, ok]).max().item() <= JET_TOL

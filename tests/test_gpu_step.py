@@ -96,19 +96,26 @@ def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True, dec_hi=
     return eng, ap, constraints, ofac, has_mu
 
 
-def _oracle_loss_grad(eng, constraints, ofac, has_mu):
+def _oracle_loss_grad(eng, constraints, ofac, has_mu, active=None):
+    """Float64 oracle loss and gradients for the engine's inputs under the
+    scheduler state `active` (default: all active): the training points are
+    those inside active subdomains (trainers.py:482-507), the takes and the
+    [active; fixed] parameter order those of get_inputs (trainers.py:332-419).
+    Returns (loss, per-layer grads over all_ims rows, extra grads, all_ims,
+    the remapped active vector, n_active)."""
     ap = eng.all_params
     req, C, L = ofac()
     p = ap["static"]["decomposition"]["subdomain"]["params"]
     xmins, xmaxs = p[0].cpu().numpy(), p[1].cpu().numpy()
     pou = ap["static"]["decomposition"]["subdomain"]["pou"].cpu().numpy().reshape(-1)
     m = xmins.shape[0]
+    active = np.ones(m, dtype=int) if active is None else np.asarray(active)
     xg = torch.cat([c[0] for c in constraints]).numpy()
     sizes = [c[0].shape[0] for c in constraints]
     offs = list(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int))
-    tips, local, cfs, cips = ot.get_x_batch(np.ones(m, dtype=int), xg, offs, sizes, xmins, xmaxs)
+    tips, local, cfs, cips = ot.get_x_batch(active, xg, offs, sizes, xmins, xmaxs)
     x = xg[tips]
-    takes, all_ims, *_ = ot.get_inputs(x, np.ones(m, dtype=int), xmins, xmaxs, pou)
+    takes, all_ims, act, active_ims, _ = ot.get_inputs(x, active, xmins, xmaxs, pou)
     takess = ot.split_takes(takes, cfs, cips, len(tips))
     cons = [[torch.tensor(c_.numpy()[loc], dtype=torch.float64) for c_ in c] for c, loc in zip(constraints, local)]
     layers = [(W.detach().cpu()[all_ims].double(), b.detach().cpu()[all_ims].double())
@@ -120,7 +127,7 @@ def _oracle_loss_grad(eng, constraints, ofac, has_mu):
     if has_mu:
         extra = (eng.pflat[:1].detach().cpu().double().reshape(()),)
     loss, gl, gx = om.value_and_grad(layers, dom, cons, takess, jmapss, C, L, extra=extra)
-    return loss, gl, gx, all_ims
+    return loss, gl, gx, all_ims, act, len(active_ims)
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -130,7 +137,7 @@ def test_loss_and_gradients(name):
           flush=True)
     loss, g, gp = eng.loss_and_grad()
     torch.cuda.synchronize()
-    oloss, ogl, ogx, all_ims = _oracle_loss_grad(eng, constraints, ofac, has_mu)
+    oloss, ogl, ogx, all_ims, _, _ = _oracle_loss_grad(eng, constraints, ofac, has_mu)
     el = abs(loss.item() - oloss.item()) / abs(oloss.item())
     print(f"  loss gpu {loss.item():.8e} oracle {oloss.item():.8e} rel {el:.2e}")
     assert el <= TOL
@@ -167,7 +174,7 @@ def test_step_with_subdomains_without_points():
     eng, ap, constraints, ofac, has_mu = _build("poisson_small", dec_hi=(2.0, 1.0))
     loss, g, _ = eng.loss_and_grad()
     torch.cuda.synchronize()
-    oloss, ogl, _, all_ims = _oracle_loss_grad(eng, constraints, ofac, False)
+    oloss, ogl, _, all_ims, _, _ = _oracle_loss_grad(eng, constraints, ofac, False)
     print(f"[empty] m={eng.m} used={len(all_ims)} loss gpu {loss.item():.8e} oracle {oloss.item():.8e}",
           flush=True)
     assert 0 < len(all_ims) < eng.m
