@@ -62,6 +62,8 @@ def checkpoint_model(i, all_params, opt_state, active, u_test_losses):
     dicts, lists, tuples, strings, numbers and numpy arrays, so that
     analysis.load_model reads it without running code from the file."""
     count, mu, nu = opt_state
+    if isinstance(count, torch.Tensor):
+        count = count.detach().cpu()
     return (int(i), _to_plain(all_params), ((np.asarray(count, dtype=np.int32).reshape(()), _to_plain(mu),
                                             _to_plain(nu)), ()),
             np.asarray(active, dtype=np.int64), np.asarray(u_test_losses, dtype=np.float64))

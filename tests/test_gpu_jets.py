@@ -190,7 +190,12 @@ def test_point_with_zero_window_sum_is_nan():
     """Boxes [0, 0.6] and [0.4, 1] span [0, 1] exactly: the points 0 and 1 lie
     on the outer edge of the only box holding them (inclusive test), whose
     window is exactly 0 there, so the reference's u / wp (trainers.py:166) is
-    0/0 = NaN in every jet column; the other points match the oracle."""
+    0/0 = NaN in every jet column.  The other three points sit where every
+    window holding them is small (1/16) and its second derivative large, so
+    the quotient rule cancels: there the float32 method itself (the oracle
+    run in float32) is 3.4e-5 off the float64 oracle in the second-derivative
+    column, and the kernel must be within 2x of that float32 error per column
+    (or 1e-5, whichever is larger)."""
     from fbpinns_amd.decompositions import RectangularDecompositionND
     from fbpinns_amd.networks import FCN
     from fbpinns_amd.trainers import _stack_subdomain_params
@@ -205,7 +210,11 @@ def test_point_with_zero_window_sum_is_nan():
     with torch.no_grad():
         ujet = FBPINNJets.apply(eng.theta, plan).cpu()
     ref, _, _ = setups.oracle_jets(ap, x, 1, 1, 1)
+    ref32, _, _ = setups.oracle_jets(ap, x, 1, 1, 1, dtype=torch.float32)
     print(f"[wp=0] gpu {ujet.tolist()} oracle {ref.tolist()}", flush=True)
     edge, ok = [0, 4], [1, 2, 3]
     assert torch.isnan(ref[:, edge]).all() and torch.isnan(ujet[:, edge]).all()
-    assert setups.col_rel_err(ujet[:, ok], ref[:, ok]).max().item() <= JET_TOL
+    e32 = setups.col_rel_err(ref32[:, ok], ref[:, ok])
+    err = setups.col_rel_err(ujet[:, ok], ref[:, ok])
+    print(f"  column errors gpu {err.tolist()} float32 oracle {e32.tolist()}", flush=True)
+    assert (err <= torch.clamp(2 * e32, min=JET_TOL)).all()
