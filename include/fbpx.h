@@ -285,6 +285,20 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
                                 const float* r, float weight, const float* g_loss, float* const* gcol,
                                 float* scratch, float* gcoef, int32_t* done);
 
+/* fbpx_residual_loss without the caller's work space: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  Same arguments and results as
+ * fbpx_residual_loss with done = NULL (the partial sums are added by a second
+ * one-workgroup launch), bit for bit.  The work space is a file-scope device
+ * array of FBPX_RESIDUAL_BLOCKS floats (4 KiB per GPU, held until the process
+ * ends), one partial sum per workgroup; the launch caps its grid at that
+ * many workgroups, so every n >= 1 fits.  No allocation, no memset, nothing
+ * that must start at zero.  Calls that overlap on one GPU (other streams or
+ * threads) share the array and race; calls in one stream's order do not. */
+int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
+                               const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
+                               float* loss);
+
 /*
  * Deterministic segmented sum, the transpose of a per-point gather
  * y[k] = v[idx[k]] of a trainable vector (a per-instance coefficient such as
