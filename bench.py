@@ -30,20 +30,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(config, kernel):
+def measured_traffic(config, kernel, launch):
     """HBM bytes per launch of `kernel` on `config`, as recorded from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes in profiles/<config>_traffic.md
-    (a line `<kernel> traffic_bytes_per_launch = N`), or None."""
+    (a line `<kernel> traffic_bytes_per_launch = N launch = <shape>`), or None
+    when no figure was recorded for this run's launch shape `launch`."""
     path = os.path.join(ROOT, "profiles", f"{config}_traffic.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(rf"^\s*{re.escape(kernel)}\s+traffic_bytes_per_launch\s*=\s*(\d+)")
+    pat = re.compile(rf"^\s*{re.escape(kernel)}\s+traffic_bytes_per_launch\s*=\s*(\d+)\s+launch\s*=\s*(.+?)\s*$")
     with open(path) as f:
         for line in f:
             m = pat.match(line)
-            if m:
+            if m and m.group(2) == launch:
                 return int(m.group(1))
     return None
+
+
+def parse_launch(text):
+    """'bins2,chunks512' -> (("bins", 2), ("chunks", 512)): forward, backward."""
+    specs = []
+    for part in text.split(","):
+        m = re.fullmatch(r"(bins|chunks)(\d+)", part.strip())
+        if not m:
+            raise SystemExit(f"--launch: {part!r} is not binsK or chunksC")
+        specs.append((m.group(1), int(m.group(2))))
+    if len(specs) != 2:
+        raise SystemExit("--launch takes two shapes: forward,backward")
+    return tuple(specs)
 
 
 def calibrate_copy(mib):
@@ -126,19 +140,45 @@ def kernel_roofline(eng, cfgmod):
                          * np.diff(wl.t["chunk_start"].cpu().numpy())).sum())
     fl_fwd = f_pair * wl.n_pairs
     fl_bwd = 2 * f_pair * trained_pairs
+    # what the matrix cores execute (the kernels' algebra runs fewer
+    # contractions than the plain jet count above)
+    n1, n2 = bin(net.d1mask).count("1"), bin(net.d2mask).count("1")
+    x_fwd, x_bwd = cfgmod.mfma_flops_per_pair(net.h, net.nh, n1, n2)
+    ex_fwd, ex_bwd = x_fwd * wl.n_pairs, x_bwd * trained_pairs
+    tf = lambda fl, t: fl / (t * 1e-3) / 1e12
     return dict(t_fwd_ms=t_fwd, t_bwd_ms=t_bwd, flop_fwd=fl_fwd, flop_bwd=fl_bwd,
-                tflops_fwd=fl_fwd / (t_fwd * 1e-3) / 1e12, tflops_bwd=fl_bwd / (t_bwd * 1e-3) / 1e12,
-                f_pair=f_pair, n_pairs=wl.n_pairs, trained_pairs=trained_pairs)
+                tflops_fwd=tf(fl_fwd, t_fwd), tflops_bwd=tf(fl_bwd, t_bwd),
+                f_pair=f_pair, n_pairs=wl.n_pairs, trained_pairs=trained_pairs,
+                executed_mfma={"flop_per_pair_fwd": x_fwd, "flop_per_pair_bwd": x_bwd,
+                               "tflops_fwd": tf(ex_fwd, t_fwd), "tflops_bwd": tf(ex_bwd, t_bwd),
+                               "frac_fwd": tf(ex_fwd, t_fwd) / PEAK_FP32_MFMA,
+                               "frac_bwd": tf(ex_bwd, t_bwd) / PEAK_FP32_MFMA})
 
 
-def cpu_baseline(c, eng, budget_s=20.0):
+def host_threads():
+    """CPUs this process may use: its affinity mask, capped by the cgroup CPU
+    quota when one is set (a container's share of a larger host)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_baseline(c, eng, budget_s=20.0, min_steps=10):
     """The oracle (torch on CPU, float32) on a bounded sample of the same
-    points, all subdomains; returns steps/s extrapolated by pair count."""
+    points -- every k-th of the (Z-ordered) training points, so the sample
+    spans the domain -- all subdomains, on every CPU this process may use;
+    times min_steps steps and returns steps/s extrapolated by pair count."""
     from oracle import adam as oadam
     from oracle import model as om
     from oracle import problems as opb
     from oracle import takes as ot
-    torch.set_num_threads(min(os.cpu_count() or 1, 16))
+    torch.set_num_threads(host_threads())
     cores = torch.get_num_threads()
     ap = eng.all_params
     p = ap["static"]["decomposition"]["subdomain"]["params"]
@@ -158,9 +198,10 @@ def cpu_baseline(c, eng, budget_s=20.0):
         return None
     xall = eng.x.cpu().numpy()
     n_full = xall.shape[0]
-    n = min(n_full, 4096)
+    n = min(n_full, 1024)
     while True:
-        x = xall[:n]
+        stride = max(1, n_full // n)
+        x = xall[::stride][:n]
         takes, all_ims, *_ = ot.get_inputs(x, np.ones(len(pou), dtype=int), xmins, xmaxs, pou)
         layers = [(W.detach().cpu()[all_ims].float(), b.detach().cpu()[all_ims].float())
                   for W, b in ap["trainable"]["network"]["subdomain"]["layers"]]
@@ -182,10 +223,11 @@ def cpu_baseline(c, eng, budget_s=20.0):
         t0 = time.time()
         layers = one_step(layers)
         t1 = time.time() - t0
-        if t1 * 3 > budget_s or n == n_full:
+        # grow the sample until min_steps steps take about the budget
+        if t1 * min_steps * 1.5 > budget_s or n == n_full:
             break
-        n = min(n_full, int(n * max(2.0, min(8.0, budget_s / (3 * max(t1, 1e-3))))))
-    steps = max(1, min(3, int(budget_s / max(t1, 1e-3))))
+        n = min(n_full, int(n * max(1.2, min(8.0, budget_s / (min_steps * max(t1, 1e-3))))))
+    steps = min_steps
     t0 = time.time()
     for _ in range(steps):
         layers = one_step(layers)
@@ -194,20 +236,32 @@ def cpu_baseline(c, eng, budget_s=20.0):
     full_pairs = eng.plan.wl.n_pairs
     rate = (s_sample / dt) / full_pairs
     return {"value": rate, "unit": "steps/s", "cores": cores, "kind": "port", "cpu_model": _cpu_model(),
-            "sample": f"oracle (torch CPU float32, nested jvp + autograd + Adam) on the first {n} of {n_full} "
-                      f"points ({s_sample} of {full_pairs} pairs), {steps} timed steps of {dt:.2f} s; "
-                      f"steps/s extrapolated by pair count"}
+            "sample": f"oracle (torch CPU float32, nested jvp + autograd + Adam) on every {stride}th of the "
+                      f"{n_full} Z-ordered points ({x.shape[0]} points, {s_sample} of {full_pairs} pairs), "
+                      f"{steps} timed steps of {dt:.2f} s on {cores} threads; steps/s extrapolated by pair count"}
 
 
 def _cpu_model():
+    """lscpu's model name and topology, with the CPUs this process may use."""
+    info = {}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    name = info.get("Model name")
+    if not name:
+        try:
+            with open("/proc/cpuinfo") as f:
+                name = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "unknown")
+        except OSError:
+            name = "unknown"
+    topo = ", ".join(f"{k} {info[k]}" for k in ("CPU(s)", "Socket(s)", "Core(s) per socket", "Thread(s) per core")
+                     if k in info)
+    return f"{name} ({topo}; {host_threads()} usable by this process)"
 
 
 def main():
@@ -224,6 +278,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch the step eagerly (for counter passes)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (counter passes see one launch shape)")
+    ap.add_argument("--launch", default=None, metavar="F,B",
+                    help="fix the forward,backward launch shapes (binsK / chunksC, e.g. bins2,bins2) "
+                         "instead of autotuning them")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
     ap.add_argument("--overlap", type=float, default=None,
@@ -235,6 +292,7 @@ def main():
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every run here uses one MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    launch = parse_launch(a.launch) if a.launch else None
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return
@@ -263,7 +321,7 @@ def main():
     eng, sched, x_test, _ = tr.setup(exact=False)
     active = next(iter(sched))
     t0 = time.time()
-    eng.set_active(active)
+    eng.set_active(active, launch=launch)
     torch.cuda.synchronize()
     log(f"work list + graph capture {time.time() - t0:.1f} s: {eng.plan.wl.n_points} points, "
         f"{eng.plan.wl.n_pairs} pairs")
@@ -286,8 +344,11 @@ def main():
     dom_is_bwd = rl["t_bwd_ms"] >= rl["t_fwd_ms"]
     ach = rl["tflops_bwd"] if dom_is_bwd else rl["tflops_fwd"]
     dom_kernel = "bwd_kernel" if dom_is_bwd else "fwd_kernel"
-    # the recorded counters are for the configuration's own overlap
-    traffic = measured_traffic(a.config, dom_kernel) if not over else None
+    # the recorded counters hold for the configuration's own overlap and the
+    # launch shape they were measured with
+    launch_txt = (f"fwd {autotune.name(eng.launch[0])} ({eng.plan.wl.n_fbins} workgroups), "
+                  f"bwd {autotune.name(eng.launch[1])} ({eng.plan.wl.n_bbins} workgroups)")
+    traffic = measured_traffic(a.config, dom_kernel, launch_txt) if not over else None
 
     plain = None
     if not a.no_plain:
@@ -350,8 +411,10 @@ def main():
         "collocation_pts_per_s": sps * n_pts,
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP32_MFMA, "traffic": traffic,
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "kernel": dom_kernel},
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated, "
+                                     "recorded for this launch shape)",
+                     "kernel": dom_kernel, "launch": launch_txt,
+                     "executed_frac": rl["executed_mfma"]["frac_bwd" if dom_is_bwd else "frac_fwd"]},
         "kernels": rl,
         "plain_gpu": plain,
         "speedup_vs_plain": (sps / plain["steps_per_s"]) if plain else None,

@@ -40,10 +40,29 @@ class fbpx_worklist(ctypes.Structure):
     ]
 
 
+class fbpx_takes_out(ctypes.Structure):
+    _fields_ = [(k, _c_p) for k in (
+        "m_take", "n_take", "p_take", "np_take", "active", "all_ims", "pair_point", "pair_seg", "pair_slot",
+        "point_seg_start", "seg_pair_start", "chunk_model", "chunk_start", "chunk_train", "row_chunk_start",
+        "fbin_chunk_start", "bbin_chunk_start")]
+
+
+class fbpx_takes_info(ctypes.Structure):
+    _fields_ = [("n_pairs", ctypes.c_int64), ("n_segs", _c_i32), ("n_used", _c_i32), ("n_active", _c_i32),
+                ("n_chunks", _c_i32), ("n_fbins", _c_i32), ("n_bbins", _c_i32), ("npou", _c_i32)]
+
+
 # name -> (restype, argtypes); mirrors include/fbpx.h
 SIGNATURES = {
     "fbpx_inside_count": (_c_i32, [_c_p, _c_p, _c_i32, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p]),
     "fbpx_inside_fill": (_c_i32, [_c_p, _c_p, _c_i32, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_p, _c_p]),
+    "fbpx_takes_max_chunks": (ctypes.c_int64, [_c_i32, ctypes.c_int64, _c_i32, _c_i32, _c_i32]),
+    "fbpx_takes_work_bytes": (ctypes.c_int64, [_c_i32, _c_i32, _c_i32, ctypes.c_int64, _c_i32, _c_i32]),
+    "fbpx_takes_count": (_c_i32, [_c_p, _c_p, _c_i32, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p, ctypes.c_int64,
+                                  ctypes.POINTER(ctypes.c_int64)]),
+    "fbpx_build_worklist": (_c_i32, [_c_p, _c_p, _c_i32, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_p,
+                                     ctypes.c_int64, _c_i32, _c_i32, _c_i32, ctypes.c_int64, _c_p, ctypes.c_int64,
+                                     ctypes.POINTER(fbpx_takes_out), ctypes.POINTER(fbpx_takes_info)]),
     "fbpx_pou_prepare": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                   _c_p, _c_p]),
     "fbpx_forward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),

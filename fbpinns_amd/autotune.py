@@ -16,7 +16,11 @@ one backward launch on each (HIP events on the current stream, after a warm-up
 launch) and returns the work list with the least forward + backward time
 among: the best chunk size for both kernels, persistent bins with each
 kernel's best multiplier, and the two mixed forms (one kernel on its best
-bins, the other on its best chunk size).  A mixed form is timed on its own
+bins, the other on its best chunk size).  Candidates within TIE (2 %) of the
+fastest count as equal -- set-up timings of the wave configuration's
+candidates move by about that much from box to box -- and among them the
+work list with the fewest chunks wins: each chunk is one partial gradient
+row that the Adam launch reads back, a cost outside the timed kernels.  A mixed form is timed on its own
 work list, since one kernel's bin edges split the other's chunks and a chunk
 cap splits the bins' chunks.  Results are cached per (network and jet
 shape, subdomain count, pair count to two significant digits), so a scheduler
@@ -39,6 +43,7 @@ BINS = [c for c in CANDIDATES if c[0] == "bins"]
 DEFAULT_SMALL = ("chunks", FBPX_CHUNK)
 DEFAULT_LARGE = ("bins", 1)
 MIN_PAIRS = 200_000
+TIE = 0.02
 _cache = {}
 
 
@@ -89,10 +94,17 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
     if key in _cache:
         fs, bs = _cache[key]
         return (fs, bs), build(fs, bs)
-    tf, tb = {}, {}
+    tf, tb, nchunks = {}, {}, {}
     for spec in CANDIDATES:
-        tf[spec], tb[spec] = _time_launches(net, theta, dom, x, build(spec, spec), reps)
-    (fs, bs), cand = select(tf, tb, lambda f, b: _time_launches(net, theta, dom, x, build(f, b), reps))
+        plan = build(spec, spec)
+        nchunks[(spec, spec)] = plan.wl.n_chunks
+        tf[spec], tb[spec] = _time_launches(net, theta, dom, x, plan, reps)
+
+    def measure(f, b):
+        plan = build(f, b)
+        nchunks[(f, b)] = plan.wl.n_chunks
+        return _time_launches(net, theta, dom, x, plan, reps)
+    (fs, bs), cand = select(tf, tb, measure, chunks=nchunks.get)
     torch.cuda.synchronize()
     _cache[key] = (fs, bs)
     mixed = [k for k in cand if k[0] != k[1] and {k[0][0], k[1][0]} == {"chunks", "bins"}]
@@ -104,15 +116,22 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
     return (fs, bs), build(fs, bs)
 
 
-def select(tf, tb, measure):
+def select(tf, tb, measure, chunks=None):
     """Launch pair (forward spec, backward spec) with the least forward +
     backward time.  tf, tb: per-candidate times measured with both kernels
     on that candidate; measure(fspec, bspec) -> (forward ms, backward ms) on
-    the mixed work list.  Returns (pair, {pair: (forward ms, backward ms)})."""
+    the mixed work list.  chunks(pair) -> the chunk count of that pair's work
+    list, or None: then the pairs within TIE of the fastest are equal and the
+    fewest chunks wins (the fastest on a tie).  Returns (pair, {pair:
+    (forward ms, backward ms)})."""
     cs = min(CHUNKS, key=lambda c: tf[c] + tb[c])
     cf, cb = min(CHUNKS, key=lambda c: tf[c]), min(CHUNKS, key=lambda c: tb[c])
     fbin, bbin = min(BINS, key=lambda c: tf[c]), min(BINS, key=lambda c: tb[c])
     cand = {(cs, cs): (tf[cs], tb[cs]), (fbin, bbin): (tf[fbin], tb[bbin])}
     for pair in ((fbin, cb), (cf, bbin)):
         cand[pair] = tuple(measure(*pair))
-    return min(cand, key=lambda k: sum(cand[k])), cand
+    best = min(cand, key=lambda k: sum(cand[k]))
+    if chunks is None:
+        return best, cand
+    near = [k for k in cand if sum(cand[k]) <= sum(cand[best]) * (1 + TIE) and chunks(k) is not None]
+    return min(near, key=lambda k: (chunks(k), sum(cand[k]))), cand

@@ -144,3 +144,20 @@ def fwd_flops_per_pair(xd, h, nh, J):
         f += 2 * h * h * J
     f += 2 * h * J
     return f
+
+
+# matrix-core FLOPs the kernels execute per (point, subdomain) pair: the
+# layer-2 contractions run as passes of v_mfma_f32_16x16x4f32 over 16 pairs,
+# (h/16) x (h/4) instructions of 2 * 16 * 16 * 4 FLOPs per pass.  Forward: Z_0,
+# the W2^T vector when any first derivative is carried, one Z_i per axis with
+# a second derivative.  Backward: Z_0; with first derivatives the s-vector
+# pass, its W2^T pass and its dW2 column; per second-derivative axis Z_i, its
+# W2^T pass and dW2 column; W2^T G_0 and the G_0 column.  One hidden layer:
+# no matrix-core work.
+def mfma_flops_per_pair(h, nh, n1, n2):
+    if nh != 2:
+        return 0, 0
+    per_pass = (h // 16) * (h // 4) * 2 * 16 * 16 * 4 / 16
+    fwd = (1 + (n1 > 0) + n2) * per_pass
+    bwd = (3 + 3 * (n1 > 0) + 3 * n2) * per_pass
+    return fwd, bwd

@@ -25,7 +25,7 @@ from fbpinns_amd import autotune, ops
 from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
 from fbpinns_amd.epilogue import Epilogue, schema_masks
 from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
-from fbpinns_amd.takes import build_worklist, get_inputs, get_x_batch, spatial_order
+from fbpinns_amd.takes import device_plan, get_x_batch, spatial_order
 from fbpinns_amd.util.logger import logger
 
 # compiled jet schemas per input dimension (fbpx_shapes.hpp)
@@ -272,13 +272,16 @@ class FusedEngine:
             cvalues.append([x[off:off + n_c]] + [v[loc] for v in self.cglobal[ic][1:]])
             cranges.append((off, n_c))
             off += n_c
-        takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
-        n_active = int((active_ == 1).sum())
-
-        n_pairs = int(len(takes[0]))
+        # the takes and the default small-list work list in one device pass
+        small = autotune.DEFAULT_SMALL
+        plan0, takes, all_ims, active_ = device_plan(x, active, self.all_params, chunk=small[1])
+        n_active = plan0.n_active
+        n_pairs = plan0.wl.n_pairs
         slots = []
 
         def build(fspec, bspec):
+            if fspec == bspec == small:
+                return plan0
             caps = [c[1] for c in (fspec, bspec) if c[0] == "chunks"]
             fb = bb = None
             if "bins" in (fspec[0], bspec[0]):
@@ -288,8 +291,8 @@ class FusedEngine:
                     slots.extend([max(1, fo) * n_cu, max(1, bo) * n_cu])
                 fb = fspec[1] * slots[0] if fspec[0] == "bins" else None
                 bb = bspec[1] * slots[1] if bspec[0] == "bins" else None
-            return build_worklist(takes, all_ims, n_active, x.shape[0], self.device,
-                                  chunk=min(caps) if caps else None, fbins=fb, bbins=bb)
+            return device_plan(x, active, self.all_params, chunk=min(caps) if caps else None, fbins=fb,
+                               bbins=bb)[0]
         if chunk is not None:
             launch = (("chunks", int(chunk)), ("chunks", int(chunk)))
         if launch is not None:
@@ -398,9 +401,8 @@ class FusedEngine:
         `active`) used and every used subdomain trained."""
         x = x.to(self.device, torch.float32).contiguous()
         active = np.ones(self.m, dtype=int) if active is None else np.asarray(active)
-        takes, all_ims, (active_, *_) = get_inputs(x, active, self.all_params, self.decomposition)
+        plan, takes, _, _ = device_plan(x, active, self.all_params)
         net = self.net0.with_schema(*schema)
-        plan = build_worklist(takes, all_ims, int((active_ == 1).sum()), x.shape[0], self.device)
         wsum = torch.empty((max(plan.wl.n_segs, 1), net.J), dtype=torch.float32, device=self.device)
         scratch = torch.empty((max(plan.wl.n_pairs, 1), net.J), dtype=torch.float32, device=self.device)
         ops.pou_prepare(net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)

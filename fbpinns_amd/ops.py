@@ -522,6 +522,64 @@ def inside_takes(x, xmins, xmaxs, mask=None):
     return n_take, m_take, counts
 
 
+def device_worklist(x, xmins, xmaxs, active, pou, chunk=0, fbins=0, bbins=0):
+    """Takes and work list of points x (n, xd) for boxes xmins / xmaxs (m, xd)
+    under the scheduler state `active` (m,) with levels `pou` (m,), built on
+    the GPU by fbpx_takes_count + fbpx_build_worklist (get_inputs,
+    trainers.py:332-419, and the fbpx_worklist layout).  chunk / fbins /
+    bbins: 0 = none (see include/fbpx.h).  Returns (tensors, info): a dict of
+    int32 CUDA tensors cut to their exact sizes and the info sizes as a dict."""
+    dev = x.device
+    x = x.to(torch.float32).contiguous()
+    xmins = xmins.to(device=dev, dtype=torch.float32).contiguous()
+    xmaxs = xmaxs.to(device=dev, dtype=torch.float32).contiguous()
+    active = torch.as_tensor(active).to(device=dev, dtype=torch.int32).contiguous()
+    pou = torch.as_tensor(pou).to(device=dev, dtype=torch.int32).reshape(-1).contiguous()
+    n, xd = x.shape
+    m = xmins.shape[0]
+    if xmins.shape != (m, xd) or xmaxs.shape != (m, xd) or active.numel() != m or pou.numel() != m:
+        raise FbpxError(f"device_worklist: shapes x {tuple(x.shape)} boxes {tuple(xmins.shape)} / "
+                        f"{tuple(xmaxs.shape)} active {active.numel()} pou {pou.numel()}")
+    L = lib()
+    st = _stream()
+    w0 = L.fbpx_takes_work_bytes(n, xd, m, 0, 0, 0)
+    if w0 < 0:
+        raise FbpxError(f"device_worklist: bad sizes n={n} xd={xd} m={m}")
+    work = torch.empty(w0, dtype=torch.uint8, device=dev)
+    offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    s_c = ctypes.c_int64()
+    check(L.fbpx_takes_count(st, _p(x, torch.float32, "x") if n else None, n, xd, xmins.data_ptr(),
+                             xmaxs.data_ptr(), m, offsets.data_ptr(), work.data_ptr(), w0, ctypes.byref(s_c)),
+          "fbpx_takes_count")
+    s = int(s_c.value)
+    max_chunks = int(L.fbpx_takes_max_chunks(m, s, chunk, fbins, bbins))
+    wb = L.fbpx_takes_work_bytes(n, xd, m, s, fbins, bbins)
+    del work
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    i32 = lambda k: torch.empty(max(int(k), 1), dtype=torch.int32, device=dev)
+    t = {"m_take": i32(s), "n_take": i32(s), "p_take": i32(s), "np_take": i32(s), "active": i32(m),
+         "all_ims": i32(m), "pair_point": i32(s), "pair_seg": i32(s), "pair_slot": i32(s),
+         "point_seg_start": i32(n + 1), "seg_pair_start": i32(s + 1), "chunk_model": i32(max_chunks),
+         "chunk_start": i32(max_chunks + 1),
+         "chunk_train": torch.empty(max(max_chunks, 1), dtype=torch.uint8, device=dev),
+         "row_chunk_start": i32(m + 1),
+         "fbin_chunk_start": i32((fbins if fbins > 0 else max_chunks) + 1),
+         "bbin_chunk_start": i32((bbins if bbins > 0 else max_chunks) + 1)}
+    out = _lib.fbpx_takes_out(**{k: v.data_ptr() for k, v in t.items()})
+    info = _lib.fbpx_takes_info()
+    check(L.fbpx_build_worklist(st, x.data_ptr() if n else None, n, xd, xmins.data_ptr(), xmaxs.data_ptr(), m,
+                                active.data_ptr(), pou.data_ptr(), offsets.data_ptr(), s, int(chunk), int(fbins),
+                                int(bbins), max_chunks, work.data_ptr(), wb, ctypes.byref(out), ctypes.byref(info)),
+          "fbpx_build_worklist")
+    nf = {k: int(getattr(info, k)) for k, _ in _lib.fbpx_takes_info._fields_}
+    cut = {"m_take": s, "n_take": s, "p_take": s, "np_take": nf["n_segs"], "active": m, "all_ims": nf["n_used"],
+           "pair_point": s, "pair_seg": s, "pair_slot": s, "point_seg_start": n + 1,
+           "seg_pair_start": nf["n_segs"] + 1, "chunk_model": nf["n_chunks"], "chunk_start": nf["n_chunks"] + 1,
+           "chunk_train": nf["n_chunks"], "row_chunk_start": nf["n_active"] + 1,
+           "fbin_chunk_start": nf["n_fbins"] + 1, "bbin_chunk_start": nf["n_bbins"] + 1}
+    return {k: t[k][:cut[k]] for k in t}, nf
+
+
 # ------------------------------------------------------------------ autograd
 
 

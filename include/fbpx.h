@@ -108,6 +108,71 @@ int fbpx_inside_fill(void* stream, const float* x, int32_t n, int32_t xd,
                      const uint8_t* model_mask, const int64_t* offsets,
                      int32_t* n_take, int32_t* m_take);
 
+/*
+ * Takes and work list built on the device: the reference's get_inputs
+ * (trainers.py:332-419: inside test, active remap :347-357, all_ims =
+ * [active; fixed], m_take re-index, the (point, pou) jnp.unique :380-388)
+ * and this library's fbpx_worklist layout for them, without host work.  Two
+ * calls, each ending with one stream synchronisation (the sizes are data
+ * dependent; the reference syncs here too, decompositions_base.py:74):
+ *
+ *   fbpx_takes_count    inside counts of the n points over all m boxes and
+ *                       their exclusive scan into offsets[n + 1] (int64);
+ *                       returns the pair count s = offsets[n] to the host.
+ *   fbpx_build_worklist everything else, into caller-allocated arrays sized
+ *                       from s (below); returns the exact sizes in *info.
+ *
+ * x: [n][xd] the training points (trainers.py:482-507 selects them);
+ * xmin, xmax: [m][xd] f32 boxes; active: [m] int32 scheduler state in
+ * {0, 1, 2}; pou: [m] int32 level of each subdomain, in [0, 32).  chunk: cut
+ * chunks every `chunk` pairs inside a subdomain (0: only at subdomain
+ * starts); fbins / bbins: forward / backward launch bins, each an equal
+ * share of the pairs (the backward's of the trained prefix), cut at
+ * round(linspace) like numpy (0: one bin per chunk).  work: device work space
+ * of fbpx_takes_work_bytes(n, xd, m, s, fbins, bbins) bytes (with s = 0 for
+ * fbpx_takes_count).  Results are a function of the inputs alone (exclusive
+ * scans in a fixed order, no ordering by atomics); takes are bit-exact with
+ * the reference's, tile order / segments / chunks with the fbpx_worklist
+ * description.  No counterpart in the reference beyond get_inputs: its XLA
+ * program re-gathers per pair instead of keeping a work list.
+ */
+typedef struct {
+  int32_t* m_take;           /* [s]   position in all_ims of each pair's subdomain */
+  int32_t* n_take;           /* [s]   point of each pair (point-major, ascending box id) */
+  int32_t* p_take;           /* [s]   segment of each pair (jnp.unique inverse) */
+  int32_t* np_take;          /* [s]   point of each segment (first n_segs written) */
+  int32_t* active;           /* [m]   the remapped scheduler state (0, 1, 2) */
+  int32_t* all_ims;          /* [m]   [active; fixed] subdomain ids (first n_used written) */
+  int32_t* pair_point;       /* [s]   fbpx_worklist arrays, tile order */
+  int32_t* pair_seg;         /* [s] */
+  int32_t* pair_slot;        /* [s] */
+  int32_t* point_seg_start;  /* [n + 1] */
+  int32_t* seg_pair_start;   /* [s + 1] (first n_segs + 1 meaningful) */
+  int32_t* chunk_model;      /* [max_chunks] */
+  int32_t* chunk_start;      /* [max_chunks + 1] */
+  uint8_t* chunk_train;      /* [max_chunks] 1 for chunks of trained subdomains */
+  int32_t* row_chunk_start;  /* [m + 1] chunks of trained row k (first n_active + 1 written) */
+  int32_t* fbin_chunk_start; /* [fbins + 1], or [max_chunks + 1] when fbins == 0 */
+  int32_t* bbin_chunk_start; /* [bbins + 1], or [max_chunks + 1] when bbins == 0 */
+} fbpx_takes_out;
+
+typedef struct {
+  int64_t n_pairs;
+  int32_t n_segs, n_used, n_active, n_chunks, n_fbins, n_bbins, npou;
+} fbpx_takes_info;
+
+/* Upper bound of the chunk count for the chunk arrays (max_chunks). */
+int64_t fbpx_takes_max_chunks(int32_t m, int64_t n_pairs, int32_t chunk, int32_t fbins, int32_t bbins);
+/* Work-space bytes of the two calls (-1 for bad sizes). */
+int64_t fbpx_takes_work_bytes(int32_t n, int32_t xd, int32_t m, int64_t n_pairs, int32_t fbins, int32_t bbins);
+int fbpx_takes_count(void* stream, const float* x, int32_t n, int32_t xd, const float* xmin, const float* xmax,
+                     int32_t m, int64_t* offsets, void* work, int64_t work_bytes, int64_t* n_pairs);
+/* offsets and n_pairs: from fbpx_takes_count on the same x and boxes. */
+int fbpx_build_worklist(void* stream, const float* x, int32_t n, int32_t xd, const float* xmin, const float* xmax,
+                        int32_t m, const int32_t* active, const int32_t* pou, const int64_t* offsets, int64_t n_pairs,
+                        int32_t chunk, int32_t fbins, int32_t bbins, int64_t max_chunks, void* work,
+                        int64_t work_bytes, const fbpx_takes_out* out, fbpx_takes_info* info);
+
 /* ---------------------------------------------------------------- forward */
 
 /* Window jets summed per segment: wsum[seg][J] = sum over the segment's pairs

@@ -11,6 +11,7 @@ import pytest
 import torch
 
 import setups
+import worklist_ref
 from fbpinns_amd import _lib, epilogue, takes as ptakes
 from fbpinns_amd.constants import Constants, get_subdomain_ws
 from fbpinns_amd.decompositions import RectangularDecompositionND
@@ -72,7 +73,7 @@ def test_build_worklist_invariants():
     active = np.ones(m, int)
     active[::4] = 2
     takes, all_ims, act, a_ims, f_ims = ot.get_inputs(x, active, lo, hi, pou)
-    plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
+    plan = worklist_ref.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
     t = {k: v.numpy() for k, v in plan.wl.t.items()}
     m_take, n_take, p_take, np_take, npou = takes
     cs = t["chunk_start"]
@@ -124,7 +125,7 @@ def test_build_worklist_bins():
     s_train = int(counts[:len(a_ims)].sum())
     model_start = np.concatenate([[0], np.cumsum(counts)])
     for cap, nf, nb in ((None, 7, 5), (None, 1, 1), (64, 13, 3), (None, 300, 200)):
-        plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=cap, fbins=nf, bbins=nb)
+        plan = worklist_ref.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=cap, fbins=nf, bbins=nb)
         t = {k: v.numpy() for k, v in plan.wl.t.items()}
         cs, fb, bb = t["chunk_start"], t["fbin_chunk_start"], t["bbin_chunk_start"]
         n_chunks = plan.wl.n_chunks
@@ -149,7 +150,7 @@ def test_build_worklist_bins():
             assert set(t["chunk_model"][rcs[k]:rcs[k + 1]]) <= {row}
             assert cs[rcs[k + 1]] - cs[rcs[k]] == counts[k]
     # no bins: one workgroup per chunk for both kernels
-    plan = ptakes.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
+    plan = worklist_ref.build_worklist(takes, all_ims, len(a_ims), x.shape[0], "cpu", chunk=64)
     n_chunks = plan.wl.n_chunks
     assert np.array_equal(plan.wl.t["fbin_chunk_start"].numpy(), np.arange(n_chunks + 1))
     assert np.array_equal(plan.wl.t["bbin_chunk_start"].numpy(), np.arange(n_chunks + 1))
@@ -217,6 +218,16 @@ def test_autotune_select():
     tb = {c256: 7.88, c512: 7.81, b1: 7.65, b2: 7.70}
     pair, _ = A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05))
     assert pair == (b2, b1)
+    # with chunk counts: candidates within TIE of the fastest are equal and the
+    # fewest chunks wins; one outside TIE never does
+    nch = {(c512, c512): 14458, (b2, b1): 2398, (b2, c512): 15000, (c512, b1): 1900}
+    pair, cand = A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05), chunks=nch.get)
+    assert sum(cand[(c512, b1)]) <= sum(cand[(b2, b1)]) * (1 + A.TIE) and pair == (c512, b1)
+    nch[(c512, b1)] = 5000
+    assert A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05), chunks=nch.get)[0] == (b2, b1)
+    slow = lambda f, b: (tf[f] + 1.0, tb[b] + 1.0)
+    nch[(c512, b1)] = 10
+    assert A.select(tf, tb, slow, chunks=nch.get)[0] == (b2, b1)
 
 
 # ------------------------------------------------------------------ epilogue

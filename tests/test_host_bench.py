@@ -1,6 +1,7 @@
 """CPU: bench.py's contract pieces that need no GPU -- more than one GPU is
 refused with a message, and the calibrated HBM traffic it reports is the
-figure recorded in the committed rocprofv3 summary."""
+figure recorded in the committed rocprofv3 summary for the run's launch
+shape (none for another shape)."""
 
 import os
 import re
@@ -23,19 +24,23 @@ def test_traffic_figure_comes_from_the_profile_summary():
     import bench
     with open(os.path.join(ROOT, "profiles", "wave_traffic.md")) as f:
         text = f.read()
-    rec = {m.group(1): int(m.group(2))
-           for m in re.finditer(r"^(\w+) traffic_bytes_per_launch = (\d+)$", text, re.M)}
+    rec = {m.group(1): (int(m.group(2)), m.group(3))
+           for m in re.finditer(r"^(\w+) traffic_bytes_per_launch = (\d+) launch = (.+)$", text, re.M)}
     assert set(rec) == {"bwd_kernel", "fwd_kernel"}
-    assert bench.measured_traffic("wave", "bwd_kernel") == rec["bwd_kernel"]
-    assert bench.measured_traffic("wave", "fwd_kernel") == rec["fwd_kernel"]
-    assert bench.measured_traffic("poisson", "bwd_kernel") is None
-    # the recorded bytes are the table's counters scaled by the copy calibration
-    rows = {m.group(1): (float(m.group(2)), float(m.group(3)))
-            for m in re.finditer(r"^\| (\w+)<[^|]*\| ([\d.]+) \| ([\d.]+) \|", text, re.M)}
-    copy = re.search(r"^\| copy \(1 GiB in, 1 GiB out\) \| ([\d.]+) \| ([\d.]+) \|", text, re.M)
+    for k, (v, launch) in rec.items():
+        assert bench.measured_traffic("wave", k, launch) == v
+        # a run with another launch shape reports no figure
+        assert bench.measured_traffic("wave", k, launch + " (other)") is None
+    assert bench.measured_traffic("poisson", "bwd_kernel", rec["bwd_kernel"][1]) is None
+    rec = {k: v for k, (v, _) in rec.items()}
+    # every recorded figure is a table row's counters scaled by the copy calibration
+    rows = [(m.group(1), float(m.group(2)), float(m.group(3)))
+            for m in re.finditer(r"^\| (\w+)<[^|]*\|[^|]*\|[^|]*\| ([\d.]+) \| ([\d.]+) \|", text, re.M)]
+    copy = re.search(r"^\| copy \(1 GiB in, 1 GiB out\) \|[^|]*\|[^|]*\| ([\d.]+) \| ([\d.]+) \|", text, re.M)
     f_fetch = 1024 * 1024 / float(copy.group(1))      # KiB fetched for 1 GiB read
     f_write = 1024 * 1024 / float(copy.group(2))
-    assert abs(f_fetch - 1.99995) < 1e-4 and f_write == 1.0
-    for k in ("bwd_kernel", "fwd_kernel"):
-        fetch, write = rows[k]
-        assert abs(round(fetch * 1.99995 * 1024 + write * f_write * 1024) - rec[k]) <= 1
+    assert abs(f_fetch - 1.99995) < 2e-4 and f_write == 1.0
+    recs = re.findall(r"^(\w+) traffic_bytes_per_launch = (\d+) launch", text, re.M)
+    assert len(recs) >= 2
+    for k, v in recs:
+        assert any(abs(round(fe * 1.99995 * 1024 + wr * f_write * 1024) - int(v)) <= 1 for kk, fe, wr in rows if kk == k), k
