@@ -104,7 +104,12 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
         plan = build(f, b)
         nchunks[(f, b)] = plan.wl.n_chunks
         return _time_launches(net, theta, dom, x, plan, reps)
-    (fs, bs), cand = select(tf, tb, measure, chunks=nchunks.get)
+
+    def chunks(pair):   # pairs not built yet (bins with two multipliers) are built here
+        if pair not in nchunks:
+            nchunks[pair] = build(*pair).wl.n_chunks
+        return nchunks[pair]
+    (fs, bs), cand = select(tf, tb, measure, chunks=chunks)
     torch.cuda.synchronize()
     _cache[key] = (fs, bs)
     mixed = [k for k in cand if k[0] != k[1] and {k[0][0], k[1][0]} == {"chunks", "bins"}]
@@ -133,5 +138,5 @@ def select(tf, tb, measure, chunks=None):
     best = min(cand, key=lambda k: sum(cand[k]))
     if chunks is None:
         return best, cand
-    near = [k for k in cand if sum(cand[k]) <= sum(cand[best]) * (1 + TIE) and chunks(k) is not None]
+    near = [k for k in cand if sum(cand[k]) <= sum(cand[best]) * (1 + TIE)]
     return min(near, key=lambda k: (chunks(k), sum(cand[k]))), cand

@@ -799,16 +799,22 @@ __device__ inline void w2t_pass(const LaneLds& v, const f4* G, f4* GA) {
 // group, 64 lanes) stays 64 consecutive floats.  At h = 64 the unpadded
 // offsets (multiples of 64 floats) let the reads pair up as ds_read2st64,
 // and the registers that saves are worth more than the store conflicts.
-// dW2 blocks (16 x 16, rows o, columns u) are owned by waves: H = 64: wave w
-// owns column block w (all 4 row blocks); H = 32: block (w >> 1, w & 1);
-// H = 16: every wave holds the single block over its own pairs.  A wave
-// contracts its blocks over the pairs of all waves (H >= 32), so each
-// accumulator is held once per workgroup instead of once per wave.
+// dW2 blocks (16 x 16, rows o, columns u): H = 64: wave w owns column block
+// w (all 4 row blocks) and contracts it over the pairs of all four waves, so
+// each accumulator is held once per workgroup instead of once per wave (the
+// whole 64 x 64 block set would not fit a wave's registers); the waves run
+// their tiles in lockstep, with workgroup barriers around each column.
+// H <= 32 (PRIV): every wave holds all NRB^2 blocks (16 registers at H = 32)
+// over its own pairs, reading back only what it wrote itself, so the tile
+// loop needs no workgroup barrier: LDS operations of one wavefront are
+// performed in order, and a wavefront-scope fence keeps the compiler from
+// moving the reads above the writes (or the next writes above the reads).
 template <int H>
 struct DW2Own {
   static constexpr int NRB = H / 16;
-  static constexpr int NOB = NRB == 4 ? 4 : 1;     // row blocks held by a wave
-  static constexpr int NSRC = NRB == 1 ? 1 : 4;    // waves whose pairs it contracts over
+  static constexpr bool PRIV = H <= 32;
+  static constexpr int NOB = PRIV ? NRB * NRB : 4;  // blocks held by a wave
+  static constexpr int NSRC = PRIV ? 1 : 4;        // waves whose pairs it contracts over
   static constexpr int PG = H == 64 ? 64 : 68;      // floats per group of 4 pairs
   static constexpr int WS = 4 * PG;                 // per wave (16 pairs)
   static constexpr int BS = 4 * WS;                 // per 16-unit block (4 waves)
@@ -823,9 +829,46 @@ struct DW2Own {
     const int c = 4 * pp + cu;
     return gp * PG + 4 * (SWZ ? c ^ (gp ^ (pp & 2)) : c);
   }
-  __device__ static int col_block(int wave) { return NRB == 4 ? wave : (NRB == 2 ? (wave & 1) : 0); }
-  __device__ static int row_block(int wave, int j) { return NRB == 4 ? j : (NRB == 2 ? (wave >> 1) : 0); }
+  __device__ static int col_block(int wave, int j) { return PRIV ? j % NRB : wave; }
+  __device__ static int row_block(int wave, int j) { return PRIV ? j / NRB : j; }
 };
+
+// Orders a wave's LDS transposes around the dW2 reads: a workgroup barrier
+// when the waves share the accumulators, else a wavefront-scope fence.
+template <bool PRIV>
+__device__ inline void tile_sync() {
+  if constexpr (PRIV)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  else
+    __syncthreads();
+}
+
+// dW2 += G_c a1_c^T over this wave's own 16 pairs (PRIV), all NRB^2 blocks.
+// stgp, stbp: this wave's G and h transposes at its lane offset (+ 16 g +
+// pl); pair group ks at + ks * PG, 16-unit block b at + b * BS.  a1_c of unit
+// u = 16 ib + pl is rebuilt from h as in dw2_shared, with sc[16 ib] for the
+// column block ib (KIND 1, 2).
+template <int H, int KIND>
+__device__ inline void dw2_private(const float* stgp, const float* stbp, const float* sc, f4* dW2) {
+  using O = DW2Own<H>;
+  constexpr int NRB = H / 16;
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    float a[NRB], b[NRB];
+#pragma unroll
+    for (int ib = 0; ib < NRB; ++ib) {
+      const float h = stbp[ib * O::BS + ks * O::PG];
+      float v = h;
+      if (KIND == 1) v = fmaf(-h, h, 1.f) * sc[16 * ib];
+      if (KIND == 2) v = h * fmaf(-h, h, 1.f) * sc[16 * ib];
+      b[ib] = v;
+    }
+#pragma unroll
+    for (int ob = 0; ob < NRB; ++ob) a[ob] = stgp[ob * O::BS + ks * O::PG];
+#pragma unroll
+    for (int j = 0; j < O::NOB; ++j) dW2[j] = mfma16(a[j / NRB], b[j % NRB], dW2[j]);
+  }
+}
 
 // dW2[j] += G_c a1_c^T over the tile.  a1_c of unit u = 16 ibw + pl is
 // rebuilt from h: KIND 0: h, 1: (1 - h^2) sc, 2: h (1 - h^2) sc with sc
@@ -880,7 +923,12 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   using O = DW2Own<H>;
   constexpr int J = S::J, NRB = H / 16, NS = 3 + XD;
   constexpr int K_WL = 0, K_B1 = 1, K_W1 = 2, K_B2 = 2 + XD;
-  constexpr int AXIS_UNROLL = H == 64 ? XD : 1;
+  // H <= 32 and XD <= 2 (wave-private dW2, registers to spare): the small
+  // per-unit gradients (dW1, dWL, db1, db2) are summed per lane over the
+  // chunk's tiles and reduced over the pair lanes once per chunk instead of
+  // once per tile (with three axes the unrolled axis loop would spill)
+  constexpr bool ACC = DW2Own<H>::PRIV && XD <= 2;
+  constexpr int AXIS_UNROLL = H == 64 || ACC ? XD : 1;
   static_assert(H >= 16 && H <= 64, "hidden width 16, 32 or 64");
   static_assert(2 * O::TSZ >= L::P, "the gradient row reuses the transpose buffers");
   __shared__ float ss[SM::N];
@@ -915,7 +963,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int pl = lane & 15, g = lane >> 4;
-  const int ucol = 16 * O::col_block(wave) + pl;   // this lane's dW2 column unit
+  const int ucol = 16 * O::col_block(wave, 0) + pl;   // this lane's dW2 column unit (shared blocks)
   const float wl_red = ss[SM::WL + 16 * (pl >> 2) + 4 * g + (pl & 3)];   // WL of the unit a reduction leaves here
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
   // both transpose buffers are swizzled at h = 64 where the four extra read
@@ -926,13 +974,26 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   float* const stgw = stg + wave * O::WS + O::template chunk_off<SWZ>(pl >> 2, pl & 3, g);
   // dW2 read bases: stgl[ks] for the G buffer; for the h buffer stbl (plain
   // layout) or stgl[ks] + stb_off (swizzled)
-  const float* const stbl = stb + O::col_block(wave) * O::BS + 16 * g + pl;
-  const int stb_off = (int)(stb - stg) + O::col_block(wave) * O::BS - (H == 32 ? (wave >> 1) * O::BS : 0);
+  const float* const stbl = stb + O::col_block(wave, 0) * O::BS + 16 * g + pl;
+  const int stb_off = (int)(stb - stg) + O::col_block(wave, 0) * O::BS;
   const float* stgl[4];
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks)
-    stgl[ks] = stg + (H == 32 ? (wave >> 1) * O::BS : 0) + O::template chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3);
+  for (int ks = 0; ks < 4; ++ks) stgl[ks] = stg + O::template chunk_off<SWZ>(ks, g, pl >> 2) + (pl & 3);
+  // PRIV: this wave's own transposes at its lane offset
+  const float* const stgp = stg + wave * O::WS + 16 * g + pl;
+  const float* const stbp = stb + wave * O::WS + 16 * g + pl;
   float* mysmall = ssmall + wave * NS * H;
+  float aW1[ACC ? XD : 1][NRB][4], aWL[NRB][4], aB1[NRB][4], aB2[NRB][4];
+  if constexpr (ACC) {
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int i = 0; i < XD; ++i) aW1[i][rb][r] = 0.f;
+        aWL[rb][r] = aB1[rb][r] = aB2[rb][r] = 0.f;
+      }
+  }
   f4 dW2[O::NOB];
 #pragma unroll
   for (int j = 0; j < O::NOB; ++j) dW2[j] = f4{0.f, 0.f, 0.f, 0.f};
@@ -976,7 +1037,7 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     layer1_stream<XD, H, SM>(v, xh, h1);
     float vWL[NRB][4], gz10[NRB][4], vv[NRB][4];
 
-    __syncthreads();   // the previous tile's transposes have been consumed
+    tile_sync<O::PRIV>();   // the previous tile's transposes have been consumed
 
     // ---- Z_0 = W2 h1 + b2 -> h2
     float h2[NRB][4];
@@ -1041,9 +1102,12 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) gz10[rb][r] = vt[rb][r] * rv[rb][r];
-      __syncthreads();
-      dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
-      __syncthreads();
+      tile_sync<O::PRIV>();
+      if constexpr (O::PRIV)
+        dw2_private<H, 0>(stgp, stbp, nullptr, dW2);
+      else
+        dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
+      tile_sync<O::PRIV>();
     }
     // layer-1 h for the B operands of the remaining dW2 columns (published
     // with the first column's G; every column has a barrier before its reads)
@@ -1113,9 +1177,12 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
               gz10[rb][r] = fmaf(p, GA[rb][r], gz10[rb][r]);
             }
         }
-        __syncthreads();
-        dw2_shared<H, 1, SWZ>(stgl, stbl, stb_off, szt[zrow + ucol], dW2, wave);
-        __syncthreads();
+        tile_sync<O::PRIV>();
+        if constexpr (O::PRIV)
+          dw2_private<H, 1>(stgp, stbp, szt + zrow + pl, dW2);
+        else
+          dw2_shared<H, 1, SWZ>(stgl, stbl, stb_off, szt[zrow + ucol], dW2, wave);
+        tile_sync<O::PRIV>();
       } else {
 #pragma unroll
         for (int rb = 0; rb < NRB; ++rb)
@@ -1123,7 +1190,14 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
           for (int r = 0; r < 4; ++r) gzi[rb][r] = e1[rb][r] * (gur_a * vt[rb][r]);
       }
       // dW1[:, i] through z1_i = zt_i (the 1 / sd_i factor after the sum)
-      reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
+      if constexpr (ACC) {
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) aW1[i][rb][r] = fmaf(gzi[rb][r], isd_i, aW1[i][rb][r]);
+      } else {
+        reduce_units_to_lds<NRB>(gzi, mysmall + (K_W1 + i) * H, pl, g, isd_i);
+      }
     }
     if constexpr (S::N1 > 0) {
       opaque<NRB>(h2);
@@ -1144,11 +1218,18 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     const bool valid_n = kn < end;
     const int pn = wl.pair_point[valid_n ? kn : beg];
     const int segn = wl.pair_seg[valid_n ? kn : beg];
+    if constexpr (ACC) {
 #pragma unroll
-    for (int ob = 0; ob < NRB; ++ob)
+      for (int ob = 0; ob < NRB; ++ob)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) vv[ob][r] = G0[ob][r];
-    reduce_units_to_lds<NRB>(vv, mysmall + K_B2 * H, pl, g, wl_red);
+        for (int r = 0; r < 4; ++r) aB2[ob][r] += G0[ob][r];
+    } else {
+#pragma unroll
+      for (int ob = 0; ob < NRB; ++ob)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) vv[ob][r] = G0[ob][r];
+      reduce_units_to_lds<NRB>(vv, mysmall + K_B2 * H, pl, g, wl_red);
+    }
     store_tile<H>(stgw, G0);
     {
       f4 GA[NRB];
@@ -1159,43 +1240,67 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
         for (int r = 0; r < 4; ++r) gz10[rb][r] = fmaf(e1[rb][r], GA[rb][r], gz10[rb][r]);
     }
     // dW1[:, i] += gz1_0 xh_i (the gz1_i parts were added per axis)
-#pragma unroll
-    for (int i = 0; i < XD; ++i) {
+    if constexpr (ACC) {
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) vv[rb][r] = gz10[rb][r] * xh[i];
-      reduce_units_to_lds<NRB>(vv, mysmall + (K_W1 + i) * H, pl, g);
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+          for (int i = 0; i < XD; ++i) aW1[i][rb][r] = fmaf(gz10[rb][r], xh[i], aW1[i][rb][r]);
+          aWL[rb][r] += vWL[rb][r];
+          aB1[rb][r] += gz10[rb][r];
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < XD; ++i) {
+#pragma unroll
+        for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) vv[rb][r] = gz10[rb][r] * xh[i];
+        reduce_units_to_lds<NRB>(vv, mysmall + (K_W1 + i) * H, pl, g);
+      }
+      reduce_units_to_lds<NRB>(vWL, mysmall + K_WL * H, pl, g);
+      reduce_units_to_lds<NRB>(gz10, mysmall + K_B1 * H, pl, g);
     }
-    reduce_units_to_lds<NRB>(vWL, mysmall + K_WL * H, pl, g);
-    reduce_units_to_lds<NRB>(gz10, mysmall + K_B1 * H, pl, g);
     // the next tile's point and upstream jets, in flight during the last dW2 column
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)pn * XD + i];
 #pragma unroll
     for (int c = 0; c < J; ++c) gUn[c] = valid_n ? gseg[(size_t)segn * J + c] : 0.f;
-    __syncthreads();
-    dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
+    tile_sync<O::PRIV>();
+    if constexpr (O::PRIV)
+      dw2_private<H, 0>(stgp, stbp, nullptr, dW2);
+    else
+      dw2_shared<H, 0, SWZ>(stgl, stbl, stb_off, 1.f, dW2, wave);
     // (the next tile's leading barrier orders these reads before new writes)
   }
 
   // ---- end of chunk: the transpose buffers become the gradient row
+  if constexpr (ACC) {
+#pragma unroll
+    for (int i = 0; i < XD; ++i) reduce_units_to_lds<NRB>(aW1[i], mysmall + (K_W1 + i) * H, pl, g);
+    reduce_units_to_lds<NRB>(aWL, mysmall + K_WL * H, pl, g);
+    reduce_units_to_lds<NRB>(aB1, mysmall + K_B1 * H, pl, g);
+    reduce_units_to_lds<NRB>(aB2, mysmall + K_B2 * H, pl, g, wl_red);
+  }
   if (g == 0) sbl[wave * 16 + pl] = dbl;
   __syncthreads();
   float* sacc = strans;
   for (int i = threadIdx.x; i < L::P; i += blockDim.x) sacc[i] = 0.f;
   __syncthreads();
-  // H >= 32: the waves hold disjoint dW2 blocks and write them at once;
-  // H = 16: every wave holds the one block, summed wave by wave
-  for (int w = 0; w < (O::NSRC == 1 ? nw : 1); ++w) {
-    if (O::NSRC != 1 || wave == w) {
+  // H = 64: the waves hold disjoint dW2 blocks and write them at once;
+  // H <= 32: every wave holds every block over its own pairs, summed wave by
+  // wave in a fixed order
+  for (int w = 0; w < (O::PRIV ? nw : 1); ++w) {
+    if (!O::PRIV || wave == w) {
 #pragma unroll
       for (int j = 0; j < O::NOB; ++j) {
         const int ob = O::row_block(wave, j);
+        const int col = 16 * O::col_block(wave, j) + pl;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          sacc[L::W2 + (16 * ob + 4 * g + r) * H + ucol] += dW2[j][r] * v.vec[SM::WL + 16 * ob + r];
+          sacc[L::W2 + (16 * ob + 4 * g + r) * H + col] += dW2[j][r] * v.vec[SM::WL + 16 * ob + r];
       }
     }
     __syncthreads();

@@ -2,8 +2,10 @@
  * fbpx.h -- C ABI of libfbpx.so, the MI355X (gfx950) FBPINN train-step engine.
  *
  * Every pointer is a DEVICE pointer owned by the caller unless stated.  No
- * entry point allocates device memory, synchronises the stream or calls back
- * into the host, so a sequence of calls can be captured into a hipGraph.
+ * entry point allocates device memory or calls back into the host, and none
+ * synchronises the stream except the take builder (fbpx_takes_count,
+ * fbpx_build_worklist, whose output sizes are data dependent), so the
+ * per-step sequence of calls can be captured into a hipGraph.
  * Every call returns 0 on success and a negative code on error; the text of
  * the last error on the calling thread is returned by fbpx_last_error().
  * Nothing throws across the ABI.  `stream` is a hipStream_t passed as void*.

@@ -230,6 +230,38 @@ def test_autotune_select():
     assert A.select(tf, tb, slow, chunks=nch.get)[0] == (b2, b1)
 
 
+def test_autotune_tune_plan_builds_every_candidate_it_compares(monkeypatch):
+    """tune_plan with stand-in timings: every pair that select compares has
+    its work list built (chunk counts known), including bins with two
+    different multipliers, and the chosen pair's work list is returned."""
+    from fbpinns_amd import autotune as A
+
+    class Plan:
+        def __init__(self, f, b):
+            self.pair = (f, b)
+            self.wl = type("W", (), {"n_chunks": {"chunks": 5000, "bins": 1000}[f[0]] + 10 * f[1] + b[1]})()
+
+    built = []
+
+    def build(f, b):
+        built.append((f, b))
+        return Plan(f, b)
+    # 512-pair chunks fastest for both kernels (3.86 ms); bins (2, 1) within
+    # TIE of it (3.89 ms) with far fewer chunks
+    times = {("chunks", 256): (1.00, 3.00), ("chunks", 512): (0.98, 2.88), ("bins", 1): (1.02, 2.90),
+             ("bins", 2): (0.99, 2.95)}
+    monkeypatch.setattr(A, "_time_launches", lambda net, th, dom, x, plan, reps: (
+        times[plan.pair[0]][0], times[plan.pair[1]][1]))
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    A._cache.clear()
+    net = type("N", (), {"as_list": lambda self: [3, 64, 2, 7, 7]})()
+    (fs, bs), plan = A.tune_plan(build, net, None, None, None, 864, 7_000_000)
+    assert plan.pair == (fs, bs)
+    assert (fs, bs) == (("bins", 2), ("bins", 1))
+    assert (("bins", 2), ("bins", 1)) in built
+    A._cache.clear()
+
+
 # ------------------------------------------------------------------ epilogue
 
 def test_get_jmaps_matches_oracle():
