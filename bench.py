@@ -103,6 +103,57 @@ def train_l1(name):
     print(json.dumps(rec), flush=True)
 
 
+def matched_l1(name, steps, point_chunks, over):
+    """The fused engine and the plain per-op form from identical parameters and
+    points, `steps` train steps each under the scheduler's first state, with
+    the test L1 of both against the problem's exact solution (trainers.py:
+    727-774) at four points along the way; prints one JSON line."""
+    from fbpinns_amd import configs
+    from fbpinns_amd.plain import PlainFBPINN
+    from fbpinns_amd.trainers import FBPINNTrainer
+    from fbpinns_amd.util.logger import logger
+    for h in logger.handlers:
+        h.setStream(sys.stderr)
+    c = configs.CONFIGS[name](**over)
+    tr = FBPINNTrainer(c, write_outputs=False)
+    eng, sched, x_test, u_exact = tr.setup(exact=True)
+    active = next(iter(sched))
+    pl = PlainFBPINN(c.problem, c.decomposition, c.network, eng,
+                     point_chunks=point_chunks if len(eng.cglobal) == 1 else 1)
+    pl.set_active(active)
+    eng.set_active(active)
+    xt = x_test.to(eng.device, torch.float32).contiguous()
+    ue = u_exact.to(eng.device).reshape(-1, 1)
+    sd = ue.std().item()
+
+    def l1(u):
+        return torch.mean(torch.abs(ue - u.reshape(-1, 1))).item()
+    marks = sorted({max(1, steps * k // 4) for k in range(1, 5)})
+    hist, t_f, t_p, done = [], 0.0, 0.0, 0
+    for mk in marks:
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(mk - done):
+            eng.step()
+        torch.cuda.synchronize()
+        t_f += time.time() - t0
+        t0 = time.time()
+        for _ in range(mk - done):
+            pl.step()
+        torch.cuda.synchronize()
+        t_p += time.time() - t0
+        done = mk
+        a, b = l1(eng.model(xt)), l1(pl.model(xt))
+        hist.append({"step": mk, "l1_fused": a, "l1_plain": b, "rel_diff": abs(a - b) / b})
+        log(f"step {mk}: test L1 fused {a:.6e} plain {b:.6e} (rel diff {abs(a - b) / b:.2e})")
+    rec = {"config": name, "steps": steps, "test_points": int(xt.shape[0]), "l1_fused": hist[-1]["l1_fused"],
+           "l1_plain": hist[-1]["l1_plain"], "l1n_fused": hist[-1]["l1_fused"] / sd,
+           "l1n_plain": hist[-1]["l1_plain"] / sd, "rel_diff": hist[-1]["rel_diff"], "history": hist,
+           "fused_steps_per_s": steps / t_f, "plain_steps_per_s": steps / t_p,
+           "speedup": t_p / t_f, "plain_point_chunks": pl.point_chunks}
+    print(json.dumps(rec), flush=True)
+
+
 def time_kernel(fn, reps):
     """Average duration (ms) of fn() launched on torch's current stream,
     measured with HIP events on that stream."""
@@ -287,6 +338,9 @@ def main():
                     help="wave/poisson/burgers: subdomain overlap factor in place of the configuration's")
     ap.add_argument("--train-l1", action="store_true",
                     help="only train --config for its full step count and print its test L1 history")
+    ap.add_argument("--matched-l1", type=int, default=0, metavar="K",
+                    help="only train the fused and the plain forms side by side for K steps from the same "
+                         "start and print both test L1 errors")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every run here uses one MI355X)",
@@ -315,6 +369,9 @@ def main():
         xs = configs.CONFIGS[a.config]().decomposition_init_kwargs["subdomain_xs"]
         lo, hi = [v[0] for v in xs], [v[-1] for v in xs]
         over = dict(decomposition_init_kwargs=configs._rect(lo, hi, [len(v) for v in xs], a.overlap))
+    if a.matched_l1:
+        matched_l1(a.config, a.matched_l1, a.plain_chunks, over)
+        return
     c = configs.CONFIGS[a.config](graph=not a.no_graph, autotune=not a.no_autotune, **over)
     log(f"config {a.config}: building problem")
     tr = FBPINNTrainer(c, write_outputs=False)

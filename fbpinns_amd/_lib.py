@@ -86,15 +86,16 @@ SIGNATURES = {
                                   ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p)]),
     "fbpx_affine_jets_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_i32, _c_i32, _c_p, _c_i32,
                                            ctypes.POINTER(_c_i32), ctypes.POINTER(_c_p), _c_p]),
-    "fbpx_residual_loss": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
-                                    ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, ctypes.c_float, _c_p,
-                                    _c_p, _c_p, _c_p]),
-    "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
-                                             ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, _c_p,
-                                             ctypes.c_float, _c_p, ctypes.POINTER(_c_p), _c_p, _c_p, _c_p]),
-    "fbpx_residual_loss_noalloc": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), _c_i32,
-                                            ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, ctypes.c_float,
-                                            _c_p, _c_p]),
+    "fbpx_residual_loss": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
+                                    _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, ctypes.c_float,
+                                    _c_p, _c_p, _c_p, _c_i32, _c_p]),
+    "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p),
+                                             ctypes.POINTER(_c_p), _c_i32, ctypes.POINTER(_c_i32),
+                                             ctypes.POINTER(_c_i32), _c_p, _c_p, ctypes.c_float, _c_p,
+                                             ctypes.POINTER(_c_p), _c_p, _c_p, _c_p]),
+    "fbpx_residual_loss_noalloc": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
+                                            _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p,
+                                            ctypes.c_float, _c_p, _c_p, _c_i32]),
     "fbpx_segment_sum": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),

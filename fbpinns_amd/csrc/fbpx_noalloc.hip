@@ -50,9 +50,9 @@ int scratch_ptr(float** out) {
 
 extern "C" {
 
-int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
-                               const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                               float* loss) {
+int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                               const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                               const float* coef, float weight, float* r, float* loss, int32_t accumulate) {
   if (n < 0 || n_terms < 1 || n_terms > FBPX_MAX_TERMS) {
     fbpx::set_error("residual_loss_noalloc: n=%lld n_terms=%d (work space: %d partial sums, at most %d terms)",
                     (long long)n, n_terms, FBPX_RESIDUAL_BLOCKS, FBPX_MAX_TERMS);
@@ -63,7 +63,8 @@ int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const fl
   if (rc) return rc;
   // done = NULL: the partials are summed by a second launch, so no counter
   // has to be zero on entry
-  return fbpx_residual_loss(stream, n, n_cols, col, n_terms, fa, fb, coef, weight, r, ws, loss, nullptr);
+  return fbpx_residual_loss(stream, n, n_cols, col, col_idx, n_terms, fa, fb, coef, weight, r, ws, loss, accumulate,
+                            nullptr);
 }
 
 }  // extern "C"

@@ -510,9 +510,11 @@ __global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict
 // ------------------------------------------------------------------ fused residual loss
 
 // Column and term tables of one residual (factor index -1: the factor 1).
+// cidx[f] != NULL: column f is gathered, F(f, k) = col[f][cidx[f][k]].
 struct ResidualArgs {
   int n_cols, n_terms;
   const float* col[FBPX_MAX_REQ];
+  const int32_t* cidx[FBPX_MAX_REQ];
   float* gcol[FBPX_MAX_REQ];
   int fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
 };
@@ -526,14 +528,14 @@ constexpr int kResUnroll = 4;
 // count would be waited for before the next one is issued).
 __device__ float kResOne[1] = {1.f};
 
-// out[j] = scale * sum over the grid's workgroups b of scratch[b * ncol + j],
-// computed by the last workgroup to finish (counted on done, which it
-// re-zeroes for the next launch) in the order of sum_columns_kernel, so one
-// launch replaces the partial-sum launch and its reduction.  Every thread of
-// every workgroup calls it after thread 0 has written the workgroup's
-// partials.
-__device__ inline void last_block_sum(const float* scratch, int ncol, float scale, float* out, int32_t* done,
-                                      float* sm) {
+// out[j] = scale * sum over the grid's workgroups b of scratch[b * ncol + j]
+// (plus out[j]'s value on entry when acc), computed by the last workgroup to
+// finish (counted on done, which it re-zeroes for the next launch) in the
+// order of sum_columns_kernel, so one launch replaces the partial-sum launch
+// and its reduction.  Every thread of every workgroup calls it after thread 0
+// has written the workgroup's partials.
+__device__ inline void last_block_sum(const float* scratch, int ncol, float scale, float* out, bool acc,
+                                      int32_t* done, float* sm) {
   __shared__ int last;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -547,20 +549,29 @@ __device__ inline void last_block_sum(const float* scratch, int ncol, float scal
     float v = 0.f;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v += scratch[b * ncol + j];
     const float t = block_tree_sum(v, sm);
-    if (threadIdx.x == 0) out[j] = scale * t;
+    if (threadIdx.x == 0) out[j] = acc ? out[j] + scale * t : scale * t;
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicExch(done, 0);
 }
 
-// Term t's factor columns and index masks (all-ones for a column, 0 for the
-// factor 1, which reads kResOne[0]), uniform across the grid.
+// Term t's factor columns, index masks (all-ones for a column, 0 for the
+// factor 1, which reads kResOne[0]) and gather indices (NULL: none), uniform
+// across the grid.
 template <int NT>
 struct TermCols {
   const float* pa[NT];
   const float* pb[NT];
   int64_t ma[NT], mb[NT];
+  const int32_t* ia[NT];
+  const int32_t* ib[NT];
 };
+
+// Factor value at point k: p[k], p[idx[k]] for a gathered column, p[0] for
+// the factor 1 (mask 0).
+__device__ inline float res_factor(const float* p, const int32_t* idx, int64_t k, int64_t mask) {
+  return p[(idx ? (int64_t)idx[k] : k) & mask];
+}
 
 static_assert(FBPX_MAX_TERMS == 8, "the residual launches below instantiate 1 .. 8 terms");
 
@@ -574,6 +585,8 @@ __device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
     c.pb[t] = hb ? a.col[a.fb[t]] : kResOne;
     c.ma[t] = ha ? ~int64_t(0) : 0;
     c.mb[t] = hb ? ~int64_t(0) : 0;
+    c.ia[t] = ha ? a.cidx[a.fa[t]] : nullptr;
+    c.ib[t] = hb ? a.cidx[a.fb[t]] : nullptr;
   }
   return c;
 }
@@ -583,7 +596,7 @@ __device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
 template <int NT>
 __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
                                                        float* __restrict__ r, float* scratch, float scale,
-                                                       float* __restrict__ loss, int32_t* done) {
+                                                       float* __restrict__ loss, bool acc, int32_t* done) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const TermCols<NT> c = term_cols<NT>(a);
   float cf[NT];
@@ -599,8 +612,8 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
       const int64_t kk = k < n ? k : 0;   // past the end: read point 0, result dropped
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        xa[u][t] = c.pa[t][kk & c.ma[t]];
-        xb[u][t] = c.pb[t][kk & c.mb[t]];
+        xa[u][t] = res_factor(c.pa[t], c.ia[t], kk, c.ma[t]);
+        xb[u][t] = res_factor(c.pb[t], c.ib[t], kk, c.mb[t]);
       }
     }
 #pragma unroll
@@ -617,7 +630,7 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
   }
   const float tot = block_tree_sum(s, sm);
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
-  if (done) last_block_sum(scratch, 1, scale, loss, done, sm);
+  if (done) last_block_sum(scratch, 1, scale, loss, acc, done, sm);
 }
 
 // gcol[f][k] = s r[k] d r / d col_f, s = g_loss[0] * w2n; scratch[block *
@@ -647,8 +660,8 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
       sr[u] = r[kk];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        xa[u][t] = c.pa[t][kk & c.ma[t]];
-        xb[u][t] = c.pb[t][kk & c.mb[t]];
+        xa[u][t] = res_factor(c.pa[t], c.ia[t], kk, c.ma[t]);
+        xb[u][t] = res_factor(c.pb[t], c.ib[t], kk, c.mb[t]);
       }
     }
 #pragma unroll
@@ -683,19 +696,20 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
     if (threadIdx.x == 0) scratch[blockIdx.x * NT + t] = tot;
     __syncthreads();
   }
-  if (done) last_block_sum(scratch, NT, 1.f, gcoef, done, sm);
+  if (done) last_block_sum(scratch, NT, 1.f, gcoef, false, done, sm);
 }
 
-// out[j] = scale * sum over b < nb of scratch[b * ncol + j], j < ncol (one
-// block; thread i sums b = i, i + 256, ... in order, then a tree).
+// out[j] = scale * sum over b < nb of scratch[b * ncol + j] (+ out[j] on
+// entry when acc), j < ncol (one block; thread i sums b = i, i + 256, ... in
+// order, then a tree).
 __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restrict__ scratch, int nb, int ncol,
-                                                          float scale, float* __restrict__ out) {
+                                                          float scale, float* __restrict__ out, bool acc) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   for (int j = 0; j < ncol; ++j) {
     float v = 0.f;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) v += scratch[b * ncol + j];
     const float t = block_tree_sum(v, sm);
-    if (threadIdx.x == 0) out[j] = scale * t;
+    if (threadIdx.x == 0) out[j] = acc ? out[j] + scale * t : scale * t;
     __syncthreads();
   }
 }
@@ -1011,8 +1025,9 @@ static int residual_blocks(int64_t n) {
   return (int)(need < FBPX_RESIDUAL_BLOCKS ? need : FBPX_RESIDUAL_BLOCKS);
 }
 
-static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms, const int32_t* fa,
-                         const int32_t* fb, float* const* gcol, ResidualArgs* a) {
+static int residual_args(int64_t n, int32_t n_cols, const float* const* col, const int32_t* const* col_idx,
+                         int32_t n_terms, const int32_t* fa, const int32_t* fb, float* const* gcol,
+                         ResidualArgs* a) {
   if (n <= 0 || n_cols < 1 || n_cols > FBPX_MAX_REQ || n_terms < 1 || n_terms > FBPX_MAX_TERMS || !col || !fa ||
       !fb) {
     set_error("residual loss: bad sizes n=%lld n_cols=%d n_terms=%d (at most %d columns, %d terms)", (long long)n,
@@ -1028,6 +1043,7 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int
       return FBPX_EINVAL;
     }
     a->col[f] = col[f];
+    a->cidx[f] = col_idx ? col_idx[f] : nullptr;
     a->gcol[f] = gcol ? gcol[f] : nullptr;
   }
   for (int t = 0; t < n_terms; ++t) {
@@ -1041,11 +1057,12 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, int
   return FBPX_OK;
 }
 
-int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
-                       const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                       float* scratch, float* loss, int32_t* done) {
+int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                       const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                       const float* coef, float weight, float* r, float* scratch, float* loss, int32_t accumulate,
+                       int32_t* done) {
   ResidualArgs a;
-  int rc = residual_args(n, n_cols, col, n_terms, fa, fb, nullptr, &a);
+  int rc = residual_args(n, n_cols, col, col_idx, n_terms, fa, fb, nullptr, &a);
   if (rc) return rc;
   if (!coef || !r || !scratch || !loss) {
     set_error("residual loss: null coef / r / scratch / loss");
@@ -1055,9 +1072,11 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
   const int nb = residual_blocks(n);
 #define FBPX_RES_CASE(NT)                                                                          \
   case NT:                                                                                         \
-    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, one); \
+    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, acc, \
+                       one);                                                                                    \
     break;
   const float scale = weight / (float)n;
+  const bool acc = accumulate != 0;
   // the last-workgroup sum (a fence and an atomic per workgroup) pays off
   // against a second launch only for a few workgroups
   int32_t* const one = nb <= kSingleLaunchBlocks ? done : nullptr;
@@ -1066,16 +1085,17 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
   }
 #undef FBPX_RES_CASE
-  if (!one) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss);
+  if (!one) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss, acc);
   return hip_check(hipGetLastError(), "residual_loss");
 }
 
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
-                                int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
-                                const float* r, float weight, const float* g_loss, float* const* gcol,
-                                float* scratch, float* gcoef, int32_t* done) {
+                                const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa,
+                                const int32_t* fb, const float* coef, const float* r, float weight,
+                                const float* g_loss, float* const* gcol, float* scratch, float* gcoef,
+                                int32_t* done) {
   ResidualArgs a;
-  int rc = residual_args(n, n_cols, col, n_terms, fa, fb, gcol, &a);
+  int rc = residual_args(n, n_cols, col, col_idx, n_terms, fa, fb, gcol, &a);
   if (rc) return rc;
   if (!coef || !r || !g_loss || !scratch) {
     set_error("residual loss backward: null coef / r / g_loss / scratch");
@@ -1096,7 +1116,7 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
   }
 #undef FBPX_RES_CASE
   if (gcoef && !one)
-    hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef);
+    hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef, false);
   return hip_check(hipGetLastError(), "residual_loss_backward");
 }
 

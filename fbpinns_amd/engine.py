@@ -179,7 +179,9 @@ class FusedEngine:
         off = 0
         for k, s in zip(self.pkeys, self.pshapes):
             n = int(np.prod(s))
-            prob[k] = leaf[off:off + n].view(s)
+            # a leaf spanning pflat is a view of it: its backward is a reshape,
+            # where a slice's backward fills a zero tensor and copies into it
+            prob[k] = leaf.view(s) if n == leaf.numel() else leaf[off:off + n].view(s)
             off += n
         tr = dict(ap["trainable"])
         tr["problem"] = prob

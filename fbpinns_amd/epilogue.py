@@ -461,36 +461,47 @@ class _JetsResidualLoss(torch.autograd.Function):
     no per-column gradient tensors."""
 
     @staticmethod
-    def _columns(ep, C, i, gathered):
-        """Residual i's factor columns: jet columns of C, data columns, and
-        the gathered columns (consumed from the iterator `gathered`)."""
+    def _columns(ep, C, i, vecs):
+        """Residual i's factor columns (jet columns of C, data columns, and for
+        a gathered column the trainable vector it reads, consumed from the
+        iterator `vecs`), the gathered columns' positions and their indices."""
         ic, weight, keys, data, *_ = ep.fused[i]
         off, n_c = ep.cranges[ic]
-        return [C[c, off:off + n_c] if c is not None else (next(gathered) if k[0] == "P" else data[k])
-                for c, k in zip(ep.combined[i], keys)]
+        cols, gathered, idx = [], [], []
+        for j, (c, k) in enumerate(zip(ep.combined[i], keys)):
+            if c is not None:
+                cols.append(C[c, off:off + n_c])
+            elif k[0] == "P":
+                cols.append(next(vecs))
+                gathered.append(j)
+                idx.append(data[k][1])
+            else:
+                cols.append(data[k])
+        return cols, gathered, idx
 
     @staticmethod
     def forward(ctx, theta, plan, ep, nr, *inputs):
-        coefs, pvecs = inputs[:nr], inputs[nr:]
+        coefs, pvecs = inputs[:nr], [v.contiguous() for v in inputs[nr:]]
         net, wl = plan.net, plan.wl
         ops.forward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
         A, B = ep.affine
         C = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=theta.device)
         ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
-        # gathered columns v[idx] of the trainable vectors, in key order
+        # the trainable vectors of the gathered columns (v[idx[k]] read in the
+        # residual kernels), in key order
         pk = [data[k] for _, _, keys, data, *_ in ep.fused for k in keys if k[0] == "P"]
-        gathered = [v.index_select(0, d[1]) for v, d in zip(pvecs, pk)]
-        it = iter(gathered)
-        total, rs = None, []
+        it = iter(pvecs)
+        # every residual adds its mean into one loss scalar
+        total = torch.empty((), dtype=torch.float32, device=theta.device)
+        rs = []
         for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
-            cols = _JetsResidualLoss._columns(ep, C, i, it)
+            cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
             r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=theta.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
-            loss = torch.empty((), dtype=torch.float32, device=theta.device)
-            ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, loss, done)
+            ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done, gathered, idx,
+                              i > 0)
             rs.append(r)
-            total = loss if total is None else total + loss
-        ctx.save_for_backward(theta, C, *coefs, *rs, *gathered)
+        ctx.save_for_backward(theta, C, *coefs, *rs, *pvecs)
         ctx.plan, ctx.ep, ctx.nr, ctx.pk = plan, ep, nr, pk
         ctx.nvec = [v.numel() for v in pvecs]
         return total
@@ -499,15 +510,15 @@ class _JetsResidualLoss(torch.autograd.Function):
     def backward(ctx, g):
         theta, C, *rest = ctx.saved_tensors
         plan, ep, nr, pk = ctx.plan, ctx.ep, ctx.nr, ctx.pk
-        coefs, rs, gathered = rest[:nr], rest[nr:2 * nr], rest[2 * nr:]
+        coefs, rs, pvecs = rest[:nr], rest[nr:2 * nr], rest[2 * nr:]
         net, wl = plan.net, plan.wl
         g = g.contiguous()
-        it = iter(gathered)
-        gcoefs, gp = [], []   # gp: per gathered column, its gradient buffer
+        it = iter(pvecs)
+        gcoefs, gp = [], []   # gp: per gathered column, its per-point gradient buffer
         for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
             off, n_c = ep.cranges[ic]
             cs = ep.combined[i]
-            cols = _JetsResidualLoss._columns(ep, C, i, it)
+            cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
             gidx, gcols = [], []
             for j, (c, k) in enumerate(zip(cs, keys)):
                 if c is not None:
@@ -522,7 +533,7 @@ class _JetsResidualLoss(torch.autograd.Function):
             gcoef = torch.empty(nt, dtype=torch.float32, device=g.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=g.device)
             ops.residual_loss_backward(cols, list(fa), list(fb), coefs[i], rs[i], weight, g, gidx, gcols, scratch,
-                                       gcoef, want, done)
+                                       gcoef, want, done, gathered, idx)
             gcoefs.append(gcoef if want else None)
         # transposes of the gathers: per trainable vector, its points summed in index order
         gvecs = []

@@ -345,32 +345,49 @@ class AffineJets(torch.autograd.Function):
         return gq, None, None, None, None, None
 
 
-def _residual_tables(cols, fa, fb, n, name):
+def _residual_tables(cols, fa, fb, n, name, gathered=(), idx=()):
+    """Host tables of one residual: column pointers, per-column gather index
+    pointers (None when no column is gathered) and the factor indices.
+    gathered[j] is the column whose values are cols[gathered[j]][idx[j][k]]."""
     if not 1 <= len(cols) <= _lib.FBPX_MAX_REQ or not 1 <= len(fa) <= _lib.FBPX_MAX_TERMS or len(fb) != len(fa):
         raise FbpxError(f"{name}: {len(cols)} columns (at most {_lib.FBPX_MAX_REQ}), {len(fa)}/{len(fb)} "
                         f"term factors (at most {_lib.FBPX_MAX_TERMS})")
-    for t in cols:
+    if len(gathered) != len(idx) or any(not 0 <= f < len(cols) for f in gathered):
+        raise FbpxError(f"{name}: gathered columns {list(gathered)} with {len(idx)} index arrays")
+    for f, t in enumerate(cols):
         _p(t, torch.float32, name)
-        if t.numel() != n:
+        if f not in gathered and t.numel() != n:
             raise FbpxError(f"{name}: column of {t.numel()} values for {n} points")
     ptrs = (ctypes.c_void_p * len(cols))(*[t.data_ptr() for t in cols])
+    iptrs = None
+    if gathered:
+        iptrs = (ctypes.c_void_p * len(cols))()
+        for f, ix in zip(gathered, idx):
+            _p(ix, torch.int32, f"{name} index")
+            if ix.numel() != n:
+                raise FbpxError(f"{name}: gather index of {ix.numel()} entries for {n} points")
+            iptrs[f] = ix.data_ptr()
     a = (ctypes.c_int32 * len(fa))(*[int(v) for v in fa])
     b = (ctypes.c_int32 * len(fb))(*[int(v) for v in fb])
-    return ptrs, a, b
+    return ptrs, iptrs, a, b
 
 
 @torch.library.custom_op("fbpx::residual_loss", mutates_args=("r", "scratch", "loss", "done"))
 def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor, weight: float,
-                  r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor) -> None:
-    """done: int32 0 (left 0), the single-launch workgroup counter."""
+                  r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
+                  gathered: List[int], idx: List[torch.Tensor], accumulate: bool) -> None:
+    """done: int32 0 (left 0), the single-launch workgroup counter.
+    gathered / idx: columns read through a per-point index (their tensor is
+    the vector gathered from); accumulate: add to loss instead of writing it."""
     n = r.numel()
-    ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss")
+    ptrs, iptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss", gathered, idx)
     if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS or loss.numel() != 1:
         raise FbpxError(f"residual_loss: coef {coef.numel()} for {len(fa)} terms, scratch {scratch.numel()}, "
                         f"loss {loss.numel()}")
-    check(lib().fbpx_residual_loss(_stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
-                                   float(weight), _p(r, torch.float32, "r"), _p(scratch, torch.float32, "scratch"),
-                                   _p(loss, torch.float32, "loss"), _p(done, torch.int32, "done")),
+    check(lib().fbpx_residual_loss(_stream(), n, len(cols), ptrs, iptrs, len(fa), a, b,
+                                   _p(coef, torch.float32, "coef"), float(weight), _p(r, torch.float32, "r"),
+                                   _p(scratch, torch.float32, "scratch"), _p(loss, torch.float32, "loss"),
+                                   int(bool(accumulate)), _p(done, torch.int32, "done")),
           "fbpx_residual_loss")
 
 
@@ -378,9 +395,9 @@ def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: 
 def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor,
                            r: torch.Tensor, weight: float, g_loss: torch.Tensor, gidx: List[int],
                            gcols: List[torch.Tensor], scratch: torch.Tensor, gcoef: torch.Tensor,
-                           want_coef: bool, done: torch.Tensor) -> None:
+                           want_coef: bool, done: torch.Tensor, gathered: List[int], idx: List[torch.Tensor]) -> None:
     n = r.numel()
-    ptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward")
+    ptrs, iptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward", gathered, idx)
     if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
         raise FbpxError(f"residual_loss_backward: gradient columns {gidx} of {len(cols)}")
     if coef.numel() != len(fa) or gcoef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS * len(fa):
@@ -393,9 +410,10 @@ def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int
             raise FbpxError(f"residual_loss_backward: gradient column of {g.numel()} values for {n} points")
         gp[i] = g.data_ptr()
     check(lib().fbpx_residual_loss_backward(
-        _stream(), n, len(cols), ptrs, len(fa), a, b, _p(coef, torch.float32, "coef"), _p(r, torch.float32, "r"),
-        float(weight), _p(g_loss, torch.float32, "g_loss"), gp, _p(scratch, torch.float32, "scratch"),
-        _p(gcoef, torch.float32, "gcoef") if want_coef else None, _p(done, torch.int32, "done")),
+        _stream(), n, len(cols), ptrs, iptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
+        _p(r, torch.float32, "r"), float(weight), _p(g_loss, torch.float32, "g_loss"), gp,
+        _p(scratch, torch.float32, "scratch"), _p(gcoef, torch.float32, "gcoef") if want_coef else None,
+        _p(done, torch.int32, "done")),
         "fbpx_residual_loss_backward")
 
 
@@ -415,7 +433,7 @@ class ResidualLoss(torch.autograd.Function):
         r = torch.empty(n, dtype=torch.float32, device=dev)
         scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=dev)
         loss = torch.empty((), dtype=torch.float32, device=dev)
-        residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss, done)
+        residual_loss(cs, list(fa), list(fb), coef, float(weight), r, scratch, loss, done, [], [], False)
         ctx.save_for_backward(coef, r, done, *cs)
         ctx.weight, ctx.fa, ctx.fb = float(weight), list(fa), list(fb)
         return loss
@@ -430,7 +448,7 @@ class ResidualLoss(torch.autograd.Function):
         gcoef = torch.empty(nt, dtype=torch.float32, device=r.device)
         scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=r.device)
         residual_loss_backward(cs, ctx.fa, ctx.fb, coef, r, ctx.weight, g.contiguous(), gidx, gcols, scratch,
-                               gcoef, bool(need[0]), done)
+                               gcoef, bool(need[0]), done, [], [])
         out = [None] * len(cs)
         for i, gc in zip(gidx, gcols):
             out[i] = gc

@@ -321,10 +321,17 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * (problems.py:123-124, 204, 265, 326, 405: a weighted mean of squared
  * pointwise residuals, replacing their chain of jnp ops and jnp.mean):
  *   r[k]  = sum over t < n_terms of coef[t] * F(fa[t], k) * F(fb[t], k),
- *   F(f, k) = col[f][k] for f >= 0, 1 for f < 0 (constant terms use fa = fb = -1),
- *   loss  = weight / n * sum over k < n of r[k]^2.
+ *   F(f, k) = col[f][k] for f >= 0 (col[f][col_idx[f][k]] when col_idx and
+ *             col_idx[f] are not NULL), 1 for f < 0 (constant terms use
+ *             fa = fb = -1),
+ *   loss  = weight / n * sum over k < n of r[k]^2  (+ loss[0] on entry when
+ *           accumulate != 0: the sum of several residuals without a kernel).
  * col: HOST array of n_cols <= FBPX_MAX_REQ device pointers to n floats (the
- * constraint's jets and per-point data such as sources or targets); fa, fb:
+ * constraint's jets and per-point data such as sources or targets) or, for a
+ * gathered column, to the vector it gathers from (a per-instance trainable
+ * coefficient indexed by each point's instance); col_idx: NULL or a HOST
+ * array of n_cols device int32 index pointers (NULL entries: plain columns,
+ * the index is the point's); fa, fb:
  * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
  * on the device).  Writes r[n] and loss[0]; scratch: FBPX_RESIDUAL_BLOCKS floats.
  * done: NULL (a second one-workgroup launch sums the per-workgroup partials)
@@ -336,21 +343,25 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  */
 #define FBPX_MAX_TERMS 8
 #define FBPX_RESIDUAL_BLOCKS 1024
-int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
-                       const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                       float* scratch, float* loss, int32_t* done);
+int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                       const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                       const float* coef, float weight, float* r, float* scratch, float* loss, int32_t accumulate,
+                       int32_t* done);
 
 /* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
  * pointer to d L / d loss):
- *   gcol[f][k] = s * r[k] * d r[k] / d col[f][k]   for every f with gcol[f] != NULL,
+ *   gcol[f][k] = s * r[k] * d r[k] / d F(f, k)     for every f with gcol[f] != NULL
+ *                (per point, also for a gathered column: fbpx_segment_sum
+ *                then sums it per vector entry),
  *   gcoef[t]   = s * sum over k of r[k] * (product of term t at k)   (gcoef may be NULL).
  * gcol: HOST array of n_cols device pointers (NULL entries: per-point data,
  * no gradient); scratch: FBPX_RESIDUAL_BLOCKS * n_terms floats; done as in
  * fbpx_residual_loss (for the gcoef sums). */
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
-                                int32_t n_terms, const int32_t* fa, const int32_t* fb, const float* coef,
-                                const float* r, float weight, const float* g_loss, float* const* gcol,
-                                float* scratch, float* gcoef, int32_t* done);
+                                const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa,
+                                const int32_t* fb, const float* coef, const float* r, float weight,
+                                const float* g_loss, float* const* gcol, float* scratch, float* gcoef,
+                                int32_t* done);
 
 /* fbpx_residual_loss without the caller's work space: no counterpart in the
  * reference; for callers that cannot allocate device memory (code under
@@ -362,9 +373,9 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
  * many workgroups, so every n >= 1 fits.  No allocation, no memset, nothing
  * that must start at zero.  Calls that overlap on one GPU (other streams or
  * threads) share the array and race; calls in one stream's order do not. */
-int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const float* const* col, int32_t n_terms,
-                               const int32_t* fa, const int32_t* fb, const float* coef, float weight, float* r,
-                               float* loss);
+int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                               const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                               const float* coef, float weight, float* r, float* loss, int32_t accumulate);
 
 /*
  * Deterministic segmented sum, the transpose of a per-point gather

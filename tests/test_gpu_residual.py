@@ -200,9 +200,9 @@ def test_single_launch_sum_equals_two_launch_sum():
         loss = torch.empty((), device=dev)
         done = torch.zeros(1, dtype=torch.int32, device=dev)
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        rc = _lib.lib().fbpx_residual_loss(stream, n, 2, ptrs, 2, fa, fb, ctypes.c_void_p(coef.data_ptr()), 0.7,
-                                           ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(scratch.data_ptr()),
-                                           ctypes.c_void_p(loss.data_ptr()),
+        rc = _lib.lib().fbpx_residual_loss(stream, n, 2, ptrs, None, 2, fa, fb, ctypes.c_void_p(coef.data_ptr()),
+                                           0.7, ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(scratch.data_ptr()),
+                                           ctypes.c_void_p(loss.data_ptr()), 0,
                                            ctypes.c_void_p(done.data_ptr()) if use_done else None)
         assert rc == 0
         torch.cuda.synchronize()
@@ -212,3 +212,55 @@ def test_single_launch_sum_equals_two_launch_sum():
     print(f"[residual] two launches {out[0]!r} one launch {out[1]!r} float64 {ref!r}", flush=True)
     assert out[0] == out[1]
     assert abs(out[0] - ref) / ref < 1e-5
+
+
+@pytest.mark.parametrize("n,n_vec,use_done", [(1000, 7, True), (300_000, 256, False), (300_000, 256, True)])
+def test_gathered_column_and_accumulate(n, n_vec, use_done):
+    """A column read through a per-point index (F = v[idx[k]], the batched
+    inverse oscillators' damping per instance) and two residuals summed into
+    one loss scalar (accumulate): loss, r and the backward's per-point
+    gradient of the gathered column against float64 torch, 1e-5 relative
+    (gradient 1e-4); with and without the single-launch sum."""
+    import ctypes
+    from fbpinns_amd import _lib
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(int(n) % 97)
+    a = torch.randn(n, generator=g).to(dev)
+    b = torch.randn(n, generator=g).to(dev)
+    v = torch.randn(n_vec, generator=g).to(dev)
+    idx = torch.randint(0, n_vec, (n,), generator=g).to(dev, torch.int32)
+    coef = torch.tensor([1.0, 2.0, -0.5], device=dev)
+    # r = a + 2 v[idx] b - 0.5
+    fa, fb = [0, 2, -1], [-1, 1, -1]
+    cols = [a, b, v]
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.lib()
+    ptrs = (ctypes.c_void_p * 3)(*[c.data_ptr() for c in cols])
+    iptrs = (ctypes.c_void_p * 3)(None, None, idx.data_ptr())
+    cfa, cfb = (ctypes.c_int32 * 3)(*fa), (ctypes.c_int32 * 3)(*fb)
+    r = torch.empty(n, device=dev)
+    scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * 3, device=dev)
+    loss = torch.full((), 0.25, device=dev)       # a first residual's loss, accumulated into
+    done = torch.zeros(1, dtype=torch.int32, device=dev)
+    dptr = ctypes.c_void_p(done.data_ptr()) if use_done else None
+    assert L.fbpx_residual_loss(stream, n, 3, ptrs, iptrs, 3, cfa, cfb, ctypes.c_void_p(coef.data_ptr()), 0.6,
+                                ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(scratch.data_ptr()),
+                                ctypes.c_void_p(loss.data_ptr()), 1, dptr) == 0
+    g_loss = torch.tensor([1.3], device=dev)
+    gv = torch.empty(n, device=dev)
+    gptrs = (ctypes.c_void_p * 3)(None, None, gv.data_ptr())
+    assert L.fbpx_residual_loss_backward(stream, n, 3, ptrs, iptrs, 3, cfa, cfb, ctypes.c_void_p(coef.data_ptr()),
+                                         ctypes.c_void_p(r.data_ptr()), 0.6, ctypes.c_void_p(g_loss.data_ptr()),
+                                         gptrs, ctypes.c_void_p(scratch.data_ptr()), None, dptr) == 0
+    torch.cuda.synchronize()
+    a64, b64, v64 = a.double().cpu(), b.double().cpu(), v.double().cpu()
+    vi = v64[idx.long().cpu()]
+    rr = a64 + 2.0 * vi * b64 - 0.5
+    lref = 0.25 + 0.6 * (rr ** 2).mean().item()
+    gref = 1.3 * 2 * 0.6 / n * rr * 2.0 * b64
+    print(f"[gather] n={n} loss {loss.item():.9e} float64 {lref:.9e}", flush=True)
+    assert abs(loss.item() - lref) <= 1e-5 * abs(lref)
+    assert torch.allclose(r.double().cpu(), rr, rtol=1e-5, atol=1e-5)
+    assert ((gv.double().cpu() - gref).abs().max() / gref.abs().max()).item() <= 1e-4
+    if use_done:
+        assert int(done.item()) == 0

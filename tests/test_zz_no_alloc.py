@@ -25,11 +25,11 @@ def _call(name, n, cols, fa, fb, coef, weight, scratch=None):
     b = (ctypes.c_int32 * len(fb))(*fb)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     if name == "fbpx_residual_loss":
-        rc = L.fbpx_residual_loss(stream, n, len(cols), ptrs, len(fa), a, b, coef.data_ptr(), weight, r.data_ptr(),
-                                  scratch.data_ptr(), loss.data_ptr(), None)
+        rc = L.fbpx_residual_loss(stream, n, len(cols), ptrs, None, len(fa), a, b, coef.data_ptr(), weight,
+                                  r.data_ptr(), scratch.data_ptr(), loss.data_ptr(), 0, None)
     else:
-        rc = L.fbpx_residual_loss_noalloc(stream, n, len(cols), ptrs, len(fa), a, b, coef.data_ptr(), weight,
-                                          r.data_ptr(), loss.data_ptr())
+        rc = L.fbpx_residual_loss_noalloc(stream, n, len(cols), ptrs, None, len(fa), a, b, coef.data_ptr(), weight,
+                                          r.data_ptr(), loss.data_ptr(), 0)
     _lib.check(rc, name)
     torch.cuda.synchronize()
     return r.cpu(), loss.cpu()
@@ -67,7 +67,7 @@ def test_noalloc_rejects_bad_sizes():
     ptrs = (ctypes.c_void_p * 1)(x.data_ptr())
     a = (ctypes.c_int32 * 9)(*([0] * 9))
     for n, nt in ((0, 1), (4, 0), (4, _lib.FBPX_MAX_TERMS + 1)):
-        rc = L.fbpx_residual_loss_noalloc(stream, n, 1, ptrs, nt, a, a, x.data_ptr(), 1.0, x.data_ptr(),
-                                          x.data_ptr())
+        rc = L.fbpx_residual_loss_noalloc(stream, n, 1, ptrs, None, nt, a, a, x.data_ptr(), 1.0, x.data_ptr(),
+                                          x.data_ptr(), 0)
         assert rc == -1, (n, nt, rc)
         assert L.fbpx_last_error().decode()
