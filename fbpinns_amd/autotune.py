@@ -56,10 +56,21 @@ def _key(net, n_pairs, m):
     return tuple(net.as_list()) + (m, round(n_pairs / 10 ** e) * 10 ** e)
 
 
+MIN_TIMED_MS = 3.0
+
+
 def _time(fn, reps):
-    fn()
+    """Average ms of fn() over at least `reps` launches after a warm-up one,
+    and over at least MIN_TIMED_MS of launches for short kernels (set-up
+    timings of ~0.1 ms kernels move by several per cent between single
+    batches while the clocks ramp)."""
     s = torch.cuda.current_stream()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    fn()
+    b.record(s)
+    b.synchronize()
+    reps = max(reps, min(50, int(MIN_TIMED_MS / max(a.elapsed_time(b), 1e-3)) + 1))
     a.record(s)
     for _ in range(reps):
         fn()
