@@ -14,17 +14,18 @@ a contiguous run of chunks (see fbpx_worklist).  Two families of launch shapes:
 tune_plan() builds the work list for every candidate, times one forward and
 one backward launch on each (HIP events on the current stream, after a warm-up
 launch) and returns the work list with the least forward + backward time
-among: the best chunk size for both kernels, persistent bins with each
+among: each candidate shape for both kernels, persistent bins with each
 kernel's best multiplier, and the two mixed forms (one kernel on its best
 bins, the other on its best chunk size).  Candidates within TIE (2 %) of the
 fastest count as equal -- set-up timings of the wave configuration's
 candidates move by about that much from box to box -- and among them the
 work list with the fewest chunks wins: each chunk is one partial gradient
-row that the Adam launch reads back, a cost outside the timed kernels.  A mixed form is timed on its own
-work list, since one kernel's bin edges split the other's chunks and a chunk
-cap splits the bins' chunks.  Results are cached per (network and jet
-shape, subdomain count, pair count to two significant digits), so a scheduler
-that revisits a shape does not tune again.  Work lists below MIN_PAIRS take
+row that the Adam launch reads back, a cost outside the timed kernels.  A
+pair of different shapes is timed on its own work list, since one kernel's
+bin edges split the other's chunks and a chunk cap splits the bins' chunks.
+Results are cached per (network and jet shape, subdomain count, pair count
+to two significant digits), so a scheduler that revisits a shape does not
+tune again.  Work lists below MIN_PAIRS take
 DEFAULT_SMALL without timing: their step is launch-bound and the timings would
 be noise.
 """
@@ -116,14 +117,14 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
         nchunks[(f, b)] = plan.wl.n_chunks
         return _time_launches(net, theta, dom, x, plan, reps)
 
-    def chunks(pair):   # pairs not built yet (bins with two multipliers) are built here
+    def chunks(pair):   # every compared pair was built above; any other is built here
         if pair not in nchunks:
             nchunks[pair] = build(*pair).wl.n_chunks
         return nchunks[pair]
     (fs, bs), cand = select(tf, tb, measure, chunks=chunks)
     torch.cuda.synchronize()
     _cache[key] = (fs, bs)
-    mixed = [k for k in cand if k[0] != k[1] and {k[0][0], k[1][0]} == {"chunks", "bins"}]
+    mixed = [k for k in cand if k[0] != k[1]]
     logger.info("autotune launch shape: " + ", ".join(
         f"{name(c)}: fwd {tf[c]:.3f} ms bwd {tb[c]:.3f} ms" for c in CANDIDATES)
         + "".join(f", fwd {name(f)} + bwd {name(b)}: {cand[(f, b)][0]:.3f} + {cand[(f, b)][1]:.3f} ms"
@@ -134,18 +135,22 @@ def tune_plan(build, net, theta, dom, x, m, n_pairs, reps=3):
 
 def select(tf, tb, measure, chunks=None):
     """Launch pair (forward spec, backward spec) with the least forward +
-    backward time.  tf, tb: per-candidate times measured with both kernels
-    on that candidate; measure(fspec, bspec) -> (forward ms, backward ms) on
-    the mixed work list.  chunks(pair) -> the chunk count of that pair's work
-    list, or None: then the pairs within TIE of the fastest are equal and the
-    fewest chunks wins (the fastest on a tie).  Returns (pair, {pair:
+    backward time among every candidate for both kernels and the mixed pairs
+    (best bins per kernel; best bins with the other kernel's best chunk
+    size).  tf, tb: per-candidate times measured with both kernels on that
+    candidate; measure(fspec, bspec) -> (forward ms, backward ms) on the
+    mixed pair's own work list.  chunks(pair) -> the chunk count of that
+    pair's work list, or None: then the pairs within TIE of the fastest are
+    equal and the fewest chunks wins (the fastest on a tie).  Returns (pair, {pair:
     (forward ms, backward ms)})."""
-    cs = min(CHUNKS, key=lambda c: tf[c] + tb[c])
     cf, cb = min(CHUNKS, key=lambda c: tf[c]), min(CHUNKS, key=lambda c: tb[c])
     fbin, bbin = min(BINS, key=lambda c: tf[c]), min(BINS, key=lambda c: tb[c])
-    cand = {(cs, cs): (tf[cs], tb[cs]), (fbin, bbin): (tf[fbin], tb[bbin])}
-    for pair in ((fbin, cb), (cf, bbin)):
-        cand[pair] = tuple(measure(*pair))
+    # every uniform pair as measured; a mixed pair on its own work list, since
+    # one kernel's bin edges also cut the other kernel's chunks
+    cand = {(c, c): (tf[c], tb[c]) for c in CANDIDATES}
+    for pair in ((fbin, bbin), (fbin, cb), (cf, bbin)):
+        if pair not in cand:
+            cand[pair] = tuple(measure(*pair))
     best = min(cand, key=lambda k: sum(cand[k]))
     if chunks is None:
         return best, cand

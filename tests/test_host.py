@@ -192,10 +192,10 @@ def test_spatial_order():
 
 
 def test_autotune_select():
-    """The autotuner's choice: the least forward + backward time among one
-    chunk size for both kernels, per-kernel best bins, and the two mixed forms
-    timed on their own work lists (a mixed pair is measured, not assembled
-    from the uniform timings)."""
+    """The autotuner's choice: the least forward + backward time among every
+    candidate for both kernels and the mixed pairs (per-kernel best bins, and
+    best bins with the other kernel's best chunk size), each mixed pair timed
+    on its own work list, not assembled from the uniform timings."""
     from fbpinns_amd import autotune as A
     c256, c512, b1, b2 = ("chunks", 256), ("chunks", 512), ("bins", 1), ("bins", 2)
     # the poisson-like case: forward fastest on bins, backward on 256-pair chunks
@@ -213,21 +213,35 @@ def test_autotune_select():
     # a mixed form that measures slower than its parts loses
     pair, _ = A.select(tf, tb, lambda f, b: (0.2, 0.4))
     assert pair == (b1, b1)
-    # the wave-like case: bins for both
-    tf = {c256: 2.95, c512: 2.94, b1: 2.90, b2: 2.89}
+    # the wave-like case: forward fastest on 2x bins, backward on 1x bins.  The
+    # mixed pair is timed on its own work list (the forward's bin edges cut the
+    # backward's chunks) and loses to uniform 1x bins when that slows its
+    # backward; it wins when its own work list keeps both kernels fast
+    tf = {c256: 2.95, c512: 2.94, b1: 2.90, b2: 2.86}
     tb = {c256: 7.88, c512: 7.81, b1: 7.65, b2: 7.70}
-    pair, _ = A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05))
-    assert pair == (b2, b1)
+    mixed = {(b2, b1): (2.86, 7.82), (b2, c512): (2.87, 7.90), (c512, b1): (2.95, 7.66)}
+    asked = []
+
+    def measure(f, b):
+        asked.append((f, b))
+        return mixed[(f, b)]
+    pair, cand = A.select(tf, tb, measure)
+    assert sorted(asked) == sorted(mixed) and pair == (b1, b1)
+    assert cand[(b2, b1)] == (2.86, 7.82) and cand[(b2, b2)] == (2.86, 7.70)
+    mixed[(b2, b1)] = (2.60, 7.65)
+    assert A.select(tf, tb, lambda f, b: mixed[(f, b)])[0] == (b2, b1)
     # with chunk counts: candidates within TIE of the fastest are equal and the
     # fewest chunks wins; one outside TIE never does
-    nch = {(c512, c512): 14458, (b2, b1): 2398, (b2, c512): 15000, (c512, b1): 1900}
-    pair, cand = A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05), chunks=nch.get)
+    nch = {(c256, c256): 28000, (c512, c512): 14458, (b1, b1): 2100, (b2, b2): 2900, (b2, b1): 2398,
+           (b2, c512): 15000, (c512, b1): 1900}
+    mixed[(b2, b1)] = (2.86, 7.65)
+    pair, cand = A.select(tf, tb, lambda f, b: mixed[(f, b)], chunks=nch.get)
     assert sum(cand[(c512, b1)]) <= sum(cand[(b2, b1)]) * (1 + A.TIE) and pair == (c512, b1)
     nch[(c512, b1)] = 5000
-    assert A.select(tf, tb, lambda f, b: (tf[f] + 0.05, tb[b] + 0.05), chunks=nch.get)[0] == (b2, b1)
-    slow = lambda f, b: (tf[f] + 1.0, tb[b] + 1.0)
+    assert A.select(tf, tb, lambda f, b: mixed[(f, b)], chunks=nch.get)[0] == (b1, b1)
     nch[(c512, b1)] = 10
-    assert A.select(tf, tb, slow, chunks=nch.get)[0] == (b2, b1)
+    slow = {k: (v[0] + 1.0, v[1] + 1.0) for k, v in mixed.items()}
+    assert A.select(tf, tb, lambda f, b: slow[(f, b)], chunks=nch.get)[0] == (b1, b1)
 
 
 def test_autotune_tune_plan_builds_every_candidate_it_compares(monkeypatch):
@@ -246,8 +260,8 @@ def test_autotune_tune_plan_builds_every_candidate_it_compares(monkeypatch):
     def build(f, b):
         built.append((f, b))
         return Plan(f, b)
-    # 512-pair chunks fastest for both kernels (3.86 ms); bins (2, 1) within
-    # TIE of it (3.89 ms) with far fewer chunks
+    # 512-pair chunks fastest for both kernels (3.86 ms); 1x bins for both
+    # within TIE of it (3.92 ms) with the fewest chunks
     times = {("chunks", 256): (1.00, 3.00), ("chunks", 512): (0.98, 2.88), ("bins", 1): (1.02, 2.90),
              ("bins", 2): (0.99, 2.95)}
     monkeypatch.setattr(A, "_time_launches", lambda net, th, dom, x, plan, reps: (
@@ -257,8 +271,8 @@ def test_autotune_tune_plan_builds_every_candidate_it_compares(monkeypatch):
     net = type("N", (), {"as_list": lambda self: [3, 64, 2, 7, 7]})()
     (fs, bs), plan = A.tune_plan(build, net, None, None, None, 864, 7_000_000)
     assert plan.pair == (fs, bs)
-    assert (fs, bs) == (("bins", 2), ("bins", 1))
-    assert (("bins", 2), ("bins", 1)) in built
+    assert (fs, bs) == (("bins", 1), ("bins", 1))
+    assert (("bins", 2), ("bins", 1)) in built and (("bins", 2), ("chunks", 512)) in built
     A._cache.clear()
 
 
