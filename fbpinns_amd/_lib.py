@@ -70,7 +70,7 @@ SIGNATURES = {
     "fbpx_pou_reduce": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p,
                                  _c_p, _c_p]),
     "fbpx_pou_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p,
-                                   _c_p, _c_p]),
+                                   _c_p, _c_p, _c_p]),
     "fbpx_occupancy": (_c_i32, [ctypes.POINTER(fbpx_net), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32)]),
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                _c_p, _c_p, _c_p]),
@@ -89,6 +89,9 @@ SIGNATURES = {
     "fbpx_residual_loss": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
                                     _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p, ctypes.c_float,
                                     _c_p, _c_p, _c_p, _c_i32, _c_p]),
+    "fbpx_residual_loss_grad": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
+                                         _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p,
+                                         ctypes.c_float, _c_p, _c_p, _c_p, _c_i32, _c_p, ctypes.POINTER(_c_p)]),
     "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p),
                                              ctypes.POINTER(_c_p), _c_i32, ctypes.POINTER(_c_i32),
                                              ctypes.POINTER(_c_i32), _c_p, _c_p, ctypes.c_float, _c_p,
@@ -96,7 +99,7 @@ SIGNATURES = {
     "fbpx_residual_loss_noalloc": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
                                             _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p,
                                             ctypes.c_float, _c_p, _c_p, _c_i32]),
-    "fbpx_segment_sum": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_p]),
+    "fbpx_segment_sum": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_last_error": (ctypes.c_char_p, []),

@@ -244,18 +244,22 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
   for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = acc[c];
 }
 
+// g_scale != NULL: g_ujet is scaled by g_scale[0] as it is read (d L / d
+// loss applied to jets computed at unit upstream gradient).
 template <class S>
 __global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
                                                            const float* __restrict__ B,
                                                            const float* __restrict__ g_ujet,
+                                                           const float* __restrict__ g_scale,
                                                            float* __restrict__ gseg) {
   constexpr int J = S::J;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wl.n_points) return;
   const float inv = 1.f / (float)wl.npou;
+  const float sc = g_scale ? g_scale[0] : 1.f;
   float gq[J];
 #pragma unroll
-  for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p];
+  for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p] * sc;
   if (B != nullptr) {
     float Bj[J], gc[J];
 #pragma unroll
@@ -591,12 +595,41 @@ __device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
   return c;
 }
 
-// r[k] = sum_t coef[t] term_t(k) for NT = n_terms; scratch[block] = sum of
-// r^2 over the block's grid-stride points.
+// gcol[f][k] = sr[u] * d r / d F(f, k) at the points k = k0 + u * stride < n
+// of one loop trip, for every f with gcol[f] != NULL: d r / d F(f, .) is the
+// sum over the terms that read column f of coef times the other factor (a
+// column read by both factors of a term counts twice).  No loads here, so a
+// runtime column count costs nothing.
 template <int NT>
+__device__ inline void residual_col_grads(const ResidualArgs& a, const float* cf, const float (*xa)[NT],
+                                          const float (*xb)[NT], const float* sr, int64_t k0, int64_t stride,
+                                          int64_t n) {
+  for (int f = 0; f < a.n_cols; ++f) {
+    float* gc = a.gcol[f];
+    if (gc == nullptr) continue;
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      float g = 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if (a.fa[t] == f) g = fmaf(cf[t], xb[u][t], g);
+        if (a.fb[t] == f) g = fmaf(cf[t], xa[u][t], g);
+      }
+      if (k < n) gc[k] = sr[u] * g;
+    }
+  }
+}
+
+// r[k] = sum_t coef[t] term_t(k) for NT = n_terms; scratch[block] = sum of
+// r^2 over the block's grid-stride points.  GRAD: also the loss's column
+// gradients at unit upstream gradient, gcol[f][k] = w2n r[k] d r / d F(f, k)
+// (the arithmetic of residual_backward_kernel with g_loss = 1).
+template <int NT, bool GRAD>
 __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
                                                        float* __restrict__ r, float* scratch, float scale,
-                                                       float* __restrict__ loss, bool acc, int32_t* done) {
+                                                       float* __restrict__ loss, bool acc, int32_t* done,
+                                                       float w2n) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const TermCols<NT> c = term_cols<NT>(a);
   float cf[NT];
@@ -605,7 +638,7 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   float s = 0.f;
   for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
-    float xa[kResUnroll][NT], xb[kResUnroll][NT];
+    float xa[kResUnroll][NT], xb[kResUnroll][NT], sr[kResUnroll];
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
@@ -622,11 +655,13 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
       float v = 0.f;
 #pragma unroll
       for (int t = 0; t < NT; ++t) v = fmaf(cf[t], xa[u][t] * xb[u][t], v);
+      sr[u] = w2n * v;
       if (k < n) {
         r[k] = v;
         s = fmaf(v, v, s);
       }
     }
+    if constexpr (GRAD) residual_col_grads<NT>(a, cf, xa, xb, sr, k0, stride, n);
   }
   const float tot = block_tree_sum(s, sm);
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
@@ -671,24 +706,7 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[t] = fmaf(sr[u], xa[u][t] * xb[u][t], acc[t]);
     }
-    // d r / d col_f = sum over the terms that read col_f of coef times the
-    // other factor (a column read by both factors of a term counts twice);
-    // no loads in this loop, so a runtime column count costs nothing
-    for (int f = 0; f < a.n_cols; ++f) {
-      float* gc = a.gcol[f];
-      if (gc == nullptr) continue;
-#pragma unroll
-      for (int u = 0; u < kResUnroll; ++u) {
-        const int64_t k = k0 + u * stride;
-        float g = 0.f;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          if (a.fa[t] == f) g = fmaf(cf[t], xb[u][t], g);
-          if (a.fb[t] == f) g = fmaf(cf[t], xa[u][t], g);
-        }
-        if (k < n) gc[k] = sr[u] * g;
-      }
-    }
+    residual_col_grads<NT>(a, cf, xa, xb, sr, k0, stride, n);
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -715,17 +733,19 @@ __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restric
 }
 
 // out[s] = sum over j in [seg_start[s], seg_start[s + 1]) of x[perm[j]] (or
-// x[j]); one block per segment, fixed order.
+// x[j]), times scale[0] when scale != NULL; one block per segment, fixed
+// order.
 __global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restrict__ seg_start,
                                                           const int32_t* __restrict__ perm,
-                                                          const float* __restrict__ x, float* __restrict__ out) {
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ scale, float* __restrict__ out) {
   __shared__ float sm[4];   // one partial per wave of the 256-thread block
   const int s = blockIdx.x;
   const int j0 = seg_start[s], j1 = seg_start[s + 1];
   float v = 0.f;
   for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) v += x[perm ? perm[j] : j];
   const float t = block_tree_sum(v, sm);
-  if (threadIdx.x == 0) out[s] = t;
+  if (threadIdx.x == 0) out[s] = scale ? t * scale[0] : t;
 }
 
 __global__ void count_increment_kernel(int32_t* count) {
@@ -831,7 +851,7 @@ int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl, 
 }
 
 int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* wsum,
-                      const float* B, const float* g_ujet, float* gseg) {
+                      const float* B, const float* g_ujet, const float* g_scale, float* gseg) {
   int J, rc;
   if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
   if (wl->n_points == 0) return FBPX_OK;
@@ -840,7 +860,7 @@ int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl
 #define X(XD, D1, D2)                                                                                          \
   if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                                 \
     hipLaunchKernelGGL((pou_backward_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, wsum, B, g_ujet, \
-                       gseg);
+                       g_scale, gseg);
   FBPX_SCHEMAS(X)
 #undef X
   return hip_check(hipGetLastError(), "pou_backward");
@@ -1057,12 +1077,14 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, con
   return FBPX_OK;
 }
 
-int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col,
-                       const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
-                       const float* coef, float weight, float* r, float* scratch, float* loss, int32_t accumulate,
-                       int32_t* done) {
+// The residual loss launch; gcol != NULL: with the column gradients at unit
+// upstream gradient (residual_kernel<NT, true>).
+static int residual_loss_launch(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                                const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa,
+                                const int32_t* fb, const float* coef, float weight, float* r, float* scratch,
+                                float* loss, int32_t accumulate, int32_t* done, float* const* gcol) {
   ResidualArgs a;
-  int rc = residual_args(n, n_cols, col, col_idx, n_terms, fa, fb, nullptr, &a);
+  int rc = residual_args(n, n_cols, col, col_idx, n_terms, fa, fb, gcol, &a);
   if (rc) return rc;
   if (!coef || !r || !scratch || !loss) {
     set_error("residual loss: null coef / r / scratch / loss");
@@ -1070,16 +1092,21 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
   }
   hipStream_t s = (hipStream_t)stream;
   const int nb = residual_blocks(n);
-#define FBPX_RES_CASE(NT)                                                                          \
-  case NT:                                                                                         \
-    hipLaunchKernelGGL(residual_kernel<NT>, dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, loss, acc, \
-                       one);                                                                                    \
-    break;
   const float scale = weight / (float)n;
+  const float w2n = 2.f * weight / (float)n;   // as in the transpose's launch
   const bool acc = accumulate != 0;
   // the last-workgroup sum (a fence and an atomic per workgroup) pays off
   // against a second launch only for a few workgroups
   int32_t* const one = nb <= kSingleLaunchBlocks ? done : nullptr;
+#define FBPX_RES_CASE(NT)                                                                                     \
+  case NT:                                                                                                    \
+    if (gcol)                                                                                                 \
+      hipLaunchKernelGGL((residual_kernel<NT, true>), dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch, scale, \
+                         loss, acc, one, w2n);                                                                \
+    else                                                                                                      \
+      hipLaunchKernelGGL((residual_kernel<NT, false>), dim3(nb), dim3(256), 0, s, n, a, coef, r, scratch,     \
+                         scale, loss, acc, one, w2n);                                                         \
+    break;
   switch (n_terms) {
     FBPX_RES_CASE(1) FBPX_RES_CASE(2) FBPX_RES_CASE(3) FBPX_RES_CASE(4)
     FBPX_RES_CASE(5) FBPX_RES_CASE(6) FBPX_RES_CASE(7) FBPX_RES_CASE(8)
@@ -1087,6 +1114,26 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
 #undef FBPX_RES_CASE
   if (!one) hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, 1, scale, loss, acc);
   return hip_check(hipGetLastError(), "residual_loss");
+}
+
+int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                       const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                       const float* coef, float weight, float* r, float* scratch, float* loss, int32_t accumulate,
+                       int32_t* done) {
+  return residual_loss_launch(stream, n, n_cols, col, col_idx, n_terms, fa, fb, coef, weight, r, scratch, loss,
+                              accumulate, done, nullptr);
+}
+
+int fbpx_residual_loss_grad(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                            const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                            const float* coef, float weight, float* r, float* scratch, float* loss,
+                            int32_t accumulate, int32_t* done, float* const* gcol) {
+  if (!gcol) {
+    set_error("residual_loss_grad: null gcol table");
+    return FBPX_EINVAL;
+  }
+  return residual_loss_launch(stream, n, n_cols, col, col_idx, n_terms, fa, fb, coef, weight, r, scratch, loss,
+                              accumulate, done, gcol);
 }
 
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,
@@ -1121,13 +1168,14 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
 }
 
 int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
-                     float* out) {
+                     const float* scale, float* out) {
   if (n_seg < 0 || (n_seg > 0 && (!seg_start || !x || !out))) {
     set_error("segment_sum: n_seg=%d, null seg_start / x / out", n_seg);
     return FBPX_EINVAL;
   }
   if (n_seg == 0) return FBPX_OK;
-  hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(256), 0, (hipStream_t)stream, seg_start, perm, x, out);
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(256), 0, (hipStream_t)stream, seg_start, perm, x, scale,
+                     out);
   return hip_check(hipGetLastError(), "segment_sum");
 }
 

@@ -138,6 +138,10 @@ class Epilogue:
         # d loss / d (constrained jets): columns no residual reads stay 0
         self.gC = (torch.zeros((net.J, x.shape[0]), dtype=torch.float32, device=x.device)
                    if self.combined is not None else None)
+        # the fused loss's forward that last wrote gC at unit upstream
+        # gradient (its token), so that only its own backward reads it there
+        self._gC_owner = None
+        self._grad_wanted = False
 
     # -------------------------------------------------------------- set-up
 
@@ -348,6 +352,9 @@ class Epilogue:
             # the trainable vectors of the gathered columns, in key order
             pvecs = [torch.as_tensor(data[k][0](all_params)).reshape(-1).to(torch.float32)
                      for _, _, keys, data, *_ in self.fused for k in keys if k[0] == "P"]
+            # inside the node's forward grad mode is off; whether a backward
+            # can follow is decided here
+            self._grad_wanted = torch.is_grad_enabled() and theta.requires_grad
             return _JetsResidualLoss.apply(theta, plan, self, len(coefs), *coefs, *pvecs)
         return self.loss(FBPINNJets.apply(theta, plan), all_params)
 
@@ -458,7 +465,17 @@ class _JetsResidualLoss(torch.autograd.Function):
     [J][n] buffer -> fbpx_pou_backward with the transposed affine map ->
     fbpx_backward -> fbpx_reduce_rows.  Same kernels and arithmetic as
     FBPINNJets + AffineJets + ResidualLoss, two launches fewer per step and
-    no per-column gradient tensors."""
+    no per-column gradient tensors.
+
+    When theta's gradient is wanted and no residual coefficient is trained,
+    the forward runs fbpx_residual_loss_grad instead: the same loss launch
+    also writes d loss / d (constrained jets) at unit upstream gradient (and
+    the per-point gradients of gathered columns), and the backward starts at
+    fbpx_pou_backward, scaled by the incoming gradient there and in
+    fbpx_segment_sum: one launch per residual fewer, the same values bit for
+    bit at the step's unit seed.  The shared [J][n] buffer carries an owner
+    token; a backward that no longer owns it (another forward of this
+    epilogue ran in between) takes the transpose path above."""
 
     @staticmethod
     def _columns(ep, C, i, vecs):
@@ -494,17 +511,46 @@ class _JetsResidualLoss(torch.autograd.Function):
         # every residual adds its mean into one loss scalar
         total = torch.empty((), dtype=torch.float32, device=theta.device)
         rs = []
+        # unit-seed gradients in the loss launch (see the class docstring)
+        pregrad = ep._grad_wanted and ctx.needs_input_grad[0] and not any(ctx.needs_input_grad[4:4 + nr])
+        gp = []
         for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
             cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
             r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=theta.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
-            ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done, gathered, idx,
-                              i > 0)
+            if pregrad:
+                gidx, gcols = _JetsResidualLoss._gradient_columns(ep, i, gp)
+                ops.residual_loss_grad(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done,
+                                       gathered, idx, i > 0, gidx, gcols)
+            else:
+                ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done, gathered,
+                                  idx, i > 0)
             rs.append(r)
         ctx.save_for_backward(theta, C, *coefs, *rs, *pvecs)
         ctx.plan, ctx.ep, ctx.nr, ctx.pk = plan, ep, nr, pk
         ctx.nvec = [v.numel() for v in pvecs]
+        ctx.token = object() if pregrad else None
+        ctx.gp = gp
+        ep._gC_owner = ctx.token
         return total
+
+    @staticmethod
+    def _gradient_columns(ep, i, gp):
+        """Residual i's gradient targets: the positions of its columns that
+        take a gradient and their buffers (rows of ep.gC for jet columns, a new
+        per-point buffer appended to gp for a gathered trainable vector)."""
+        ic, _, keys, *_ = ep.fused[i]
+        off, n_c = ep.cranges[ic]
+        gidx, gcols = [], []
+        for j, (c, k) in enumerate(zip(ep.combined[i], keys)):
+            if c is not None:
+                gidx.append(j)
+                gcols.append(ep.gC[c, off:off + n_c])
+            elif k[0] == "P":
+                gidx.append(j)
+                gcols.append(torch.empty(n_c, dtype=torch.float32, device=ep.gC.device))
+                gp.append(gcols[-1])
+        return gidx, gcols
 
     @staticmethod
     def backward(ctx, g):
@@ -513,35 +559,32 @@ class _JetsResidualLoss(torch.autograd.Function):
         coefs, rs, pvecs = rest[:nr], rest[nr:2 * nr], rest[2 * nr:]
         net, wl = plan.net, plan.wl
         g = g.contiguous()
+        # the forward's unit-seed gradients are still in ep.gC: scale by g
+        # where they are read; else the transposes write them here
+        pregrad = ctx.token is not None and ep._gC_owner is ctx.token
+        ep._gC_owner = None
+        scale = g if pregrad else None
+        gp = ctx.gp if pregrad else []   # gp: per gathered column, its per-point gradient buffer
+        gcoefs = [None] * nr
         it = iter(pvecs)
-        gcoefs, gp = [], []   # gp: per gathered column, its per-point gradient buffer
-        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
-            off, n_c = ep.cranges[ic]
-            cs = ep.combined[i]
+        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused if not pregrad else ()):
             cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
-            gidx, gcols = [], []
-            for j, (c, k) in enumerate(zip(cs, keys)):
-                if c is not None:
-                    gidx.append(j)
-                    gcols.append(ep.gC[c, off:off + n_c])
-                elif k[0] == "P":
-                    gidx.append(j)
-                    gcols.append(torch.empty(n_c, dtype=torch.float32, device=g.device))
-                    gp.append(gcols[-1])
+            gidx, gcols = _JetsResidualLoss._gradient_columns(ep, i, gp)
             want = bool(ctx.needs_input_grad[4 + i])
             nt = len(fa)
             gcoef = torch.empty(nt, dtype=torch.float32, device=g.device)
             scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * nt, dtype=torch.float32, device=g.device)
             ops.residual_loss_backward(cols, list(fa), list(fb), coefs[i], rs[i], weight, g, gidx, gcols, scratch,
                                        gcoef, want, done, gathered, idx)
-            gcoefs.append(gcoef if want else None)
+            gcoefs[i] = gcoef if want else None
         # transposes of the gathers: per trainable vector, its points summed in index order
         gvecs = []
         for d, gcol, nv in zip(pk, gp, ctx.nvec):
             out = torch.empty(nv, dtype=torch.float32, device=g.device)
-            ops.segment_sum(d[3], d[2], gcol, out)
+            ops.segment_sum(d[3], d[2], gcol, out, scale)
             gvecs.append(out)
-        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1])
+        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1],
+                         scale)
         ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                      plan.gseg, plan.partial)
         return (ops.theta_gradient(plan, theta), None, None, None, *gcoefs, *gvecs)

@@ -188,14 +188,20 @@ def pou_reduce(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbu
 
 @torch.library.custom_op("fbpx::pou_backward", mutates_args=("gseg",))
 def pou_backward(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum: torch.Tensor,
-                 g_ujet: torch.Tensor, gseg: torch.Tensor, B: Optional[torch.Tensor] = None) -> None:
+                 g_ujet: torch.Tensor, gseg: torch.Tensor, B: Optional[torch.Tensor] = None,
+                 g_scale: Optional[torch.Tensor] = None) -> None:
+    """g_scale: None, or a one-element tensor by which g_ujet is multiplied
+    (d L / d loss for jets computed at unit upstream gradient)."""
     n = _net_from_list(net)
     w = _wl_struct(wl_t, wl_n)
     if B is not None and B.shape != g_ujet.shape:
         raise FbpxError(f"pou_backward: B {tuple(B.shape)} for jets {tuple(g_ujet.shape)}")
+    if g_scale is not None and g_scale.numel() != 1:
+        raise FbpxError(f"pou_backward: g_scale of {g_scale.numel()} values")
     check(lib().fbpx_pou_backward(_stream(), ctypes.byref(n), ctypes.byref(w), _p(wsum, torch.float32, "wsum"),
                                   _p(B, torch.float32, "B"), _p(g_ujet, torch.float32, "g_ujet"),
-                                  _p(gseg, torch.float32, "gseg")), "fbpx_pou_backward")
+                                  _p(g_scale, torch.float32, "g_scale"), _p(gseg, torch.float32, "gseg")),
+          "fbpx_pou_backward")
 
 
 @torch.library.custom_op("fbpx::backward", mutates_args=("partial",))
@@ -372,6 +378,20 @@ def _residual_tables(cols, fa, fb, n, name, gathered=(), idx=()):
     return ptrs, iptrs, a, b
 
 
+def _gradient_table(cols, gidx, gcols, n, name):
+    """Host table of per-column gradient pointers: gcols[j] (n floats) for
+    column gidx[j], NULL for the columns without a gradient."""
+    if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
+        raise FbpxError(f"{name}: gradient columns {gidx} of {len(cols)}")
+    gp = (ctypes.c_void_p * len(cols))()
+    for i, g in zip(gidx, gcols):
+        _p(g, torch.float32, "gcol")
+        if g.numel() != n:
+            raise FbpxError(f"{name}: gradient column of {g.numel()} values for {n} points")
+        gp[i] = g.data_ptr()
+    return gp
+
+
 @torch.library.custom_op("fbpx::residual_loss", mutates_args=("r", "scratch", "loss", "done"))
 def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor, weight: float,
                   r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
@@ -391,6 +411,26 @@ def residual_loss(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: 
           "fbpx_residual_loss")
 
 
+@torch.library.custom_op("fbpx::residual_loss_grad", mutates_args=("r", "scratch", "loss", "done", "gcols"))
+def residual_loss_grad(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor, weight: float,
+                       r: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
+                       gathered: List[int], idx: List[torch.Tensor], accumulate: bool, gidx: List[int],
+                       gcols: List[torch.Tensor]) -> None:
+    """residual_loss and, in the same launch, d loss / d cols[gidx[j]] at unit
+    upstream gradient into gcols[j] (one value per point)."""
+    n = r.numel()
+    ptrs, iptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_grad", gathered, idx)
+    if coef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS or loss.numel() != 1:
+        raise FbpxError(f"residual_loss_grad: coef {coef.numel()} for {len(fa)} terms, scratch {scratch.numel()}, "
+                        f"loss {loss.numel()}")
+    gptrs = _gradient_table(cols, gidx, gcols, n, "residual_loss_grad")
+    check(lib().fbpx_residual_loss_grad(_stream(), n, len(cols), ptrs, iptrs, len(fa), a, b,
+                                        _p(coef, torch.float32, "coef"), float(weight), _p(r, torch.float32, "r"),
+                                        _p(scratch, torch.float32, "scratch"), _p(loss, torch.float32, "loss"),
+                                        int(bool(accumulate)), _p(done, torch.int32, "done"), gptrs),
+          "fbpx_residual_loss_grad")
+
+
 @torch.library.custom_op("fbpx::residual_loss_backward", mutates_args=("gcols", "scratch", "gcoef", "done"))
 def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int], coef: torch.Tensor,
                            r: torch.Tensor, weight: float, g_loss: torch.Tensor, gidx: List[int],
@@ -398,17 +438,10 @@ def residual_loss_backward(cols: List[torch.Tensor], fa: List[int], fb: List[int
                            want_coef: bool, done: torch.Tensor, gathered: List[int], idx: List[torch.Tensor]) -> None:
     n = r.numel()
     ptrs, iptrs, a, b = _residual_tables(cols, fa, fb, n, "residual_loss_backward", gathered, idx)
-    if len(gidx) != len(gcols) or any(not 0 <= i < len(cols) for i in gidx):
-        raise FbpxError(f"residual_loss_backward: gradient columns {gidx} of {len(cols)}")
     if coef.numel() != len(fa) or gcoef.numel() != len(fa) or scratch.numel() < _lib.FBPX_RESIDUAL_BLOCKS * len(fa):
         raise FbpxError(f"residual_loss_backward: coef {coef.numel()} / gcoef {gcoef.numel()} for {len(fa)} "
                         f"terms, scratch {scratch.numel()}")
-    gp = (ctypes.c_void_p * len(cols))()
-    for i, g in zip(gidx, gcols):
-        _p(g, torch.float32, "gcol")
-        if g.numel() != n:
-            raise FbpxError(f"residual_loss_backward: gradient column of {g.numel()} values for {n} points")
-        gp[i] = g.data_ptr()
+    gp = _gradient_table(cols, gidx, gcols, n, "residual_loss_backward")
     check(lib().fbpx_residual_loss_backward(
         _stream(), n, len(cols), ptrs, iptrs, len(fa), a, b, _p(coef, torch.float32, "coef"),
         _p(r, torch.float32, "r"), float(weight), _p(g_loss, torch.float32, "g_loss"), gp,
@@ -456,14 +489,17 @@ class ResidualLoss(torch.autograd.Function):
 
 
 @torch.library.custom_op("fbpx::segment_sum", mutates_args=("out",))
-def segment_sum(seg_start: torch.Tensor, perm: torch.Tensor, x: torch.Tensor, out: torch.Tensor) -> None:
+def segment_sum(seg_start: torch.Tensor, perm: torch.Tensor, x: torch.Tensor, out: torch.Tensor,
+                scale: Optional[torch.Tensor] = None) -> None:
     n_seg = out.numel()
-    if seg_start.numel() != n_seg + 1 or perm.numel() != x.numel():
+    if seg_start.numel() != n_seg + 1 or perm.numel() != x.numel() or (scale is not None and scale.numel() != 1):
         raise FbpxError(f"segment_sum: {seg_start.numel()} offsets for {n_seg} segments, "
-                        f"{perm.numel()} permutation entries for {x.numel()} values")
+                        f"{perm.numel()} permutation entries for {x.numel()} values"
+                        + ("" if scale is None else f", scale of {scale.numel()} values"))
     check(lib().fbpx_segment_sum(_stream(), n_seg, _p(seg_start, torch.int32, "seg_start"),
                                  _p(perm, torch.int32, "perm"), _p(x, torch.float32, "x"),
-                                 _p(out, torch.float32, "out")), "fbpx_segment_sum")
+                                 _p(scale, torch.float32, "scale"), _p(out, torch.float32, "out")),
+          "fbpx_segment_sum")
 
 
 class GatherRows(torch.autograd.Function):

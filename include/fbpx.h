@@ -211,9 +211,12 @@ int fbpx_pou_reduce(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
 /* Transpose of fbpx_pou_reduce: from d loss / d ujet ([J][n_points]) to
  * d loss / d (segment sum of u*w jets), gseg: [n_segs][J].  B: NULL, or the
  * affine map's B jets as given to fbpx_pou_reduce (its transpose is applied
- * first). */
+ * first).  g_scale: NULL, or a DEVICE float by which g_ujet is multiplied as
+ * it is read (d L / d loss, when g_ujet was computed at unit upstream
+ * gradient by fbpx_residual_loss_grad). */
 int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl,
-                      const float* wsum, const float* B, const float* g_ujet, float* gseg);
+                      const float* wsum, const float* B, const float* g_ujet, const float* g_scale,
+                      float* gseg);
 
 /* Reverse pass through the jets to the subdomain weights (the part of
  * value_and_grad, trainers.py:292, below the segment sums).  Recomputes
@@ -348,6 +351,21 @@ int fbpx_residual_loss(void* stream, int64_t n, int32_t n_cols, const float* con
                        const float* coef, float weight, float* r, float* scratch, float* loss, int32_t accumulate,
                        int32_t* done);
 
+/* fbpx_residual_loss and, in the same launch, the loss's gradient with
+ * respect to its columns at unit upstream gradient (d L / d loss = 1, the
+ * train step's seed): gcol[f][k] = 2 * weight / n * r[k] * d r[k] / d F(f, k)
+ * for every f with gcol[f] != NULL, the arithmetic of
+ * fbpx_residual_loss_backward with g_loss[0] = 1, bit for bit.  The caller
+ * applies the actual d L / d loss where gcol is consumed (fbpx_pou_backward's
+ * g_scale, fbpx_segment_sum's scale), so one launch replaces the loss and its
+ * transpose when no coefficient is trained.  Same arguments as
+ * fbpx_residual_loss, plus gcol: HOST array of n_cols device pointers to n
+ * floats or NULL entries (the table itself must not be NULL). */
+int fbpx_residual_loss_grad(void* stream, int64_t n, int32_t n_cols, const float* const* col,
+                            const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
+                            const float* coef, float weight, float* r, float* scratch, float* loss,
+                            int32_t accumulate, int32_t* done, float* const* gcol);
+
 /* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
  * pointer to d L / d loss):
  *   gcol[f][k] = s * r[k] * d r[k] / d F(f, k)     for every f with gcol[f] != NULL
@@ -383,12 +401,13 @@ int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const fl
  * each oscillator's learnt damping; the reverse mode of jax's gather,
  * a scatter-add, which the reference would run for such a loss):
  *   out[s] = sum over j in [seg_start[s], seg_start[s+1]) of x[perm[j]]
- * (perm NULL: x[j]) for s < n_seg, in increasing j, no atomics.  With perm a
- * stable argsort of idx and seg_start its segment offsets, out = gather^T x.
+ * (perm NULL: x[j]) for s < n_seg, in increasing j, no atomics, times
+ * scale[0] when scale (DEVICE float) is not NULL.  With perm a stable
+ * argsort of idx and seg_start its segment offsets, out = gather^T x.
  * seg_start: DEVICE, n_seg + 1 int32; perm: DEVICE int32 or NULL.
  */
 int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
-                     float* out);
+                     const float* scale, float* out);
 
 /* ---------------------------------------------------------------- misc */
 

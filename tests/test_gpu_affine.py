@@ -86,7 +86,8 @@ def test_pou_reduce_with_affine_map_equals_separate_kernels(prob, kw, xd, lo, hi
     """fbpx_pou_reduce with the affine jets A, B equals fbpx_pou_reduce
     followed by fbpx_affine_jets on every column (bit for bit: same
     operations), and fbpx_pou_backward with B equals fbpx_affine_jets_backward
-    followed by fbpx_pou_backward."""
+    followed by fbpx_pou_backward; its g_scale equals scaling the incoming
+    jets first, bit for bit."""
     import numpy as np
     import setups
     from fbpinns_amd import ops
@@ -135,3 +136,12 @@ def test_pou_reduce_with_affine_map_equals_separate_kernels(prob, kw, xd, lo, hi
     e = ((gseg_a - gseg_b).abs().max() / gseg_b.abs().max()).item()
     print(f"  forward bit-exact, backward rel {e:.1e}", flush=True)
     assert e < 1e-6
+    # g_scale multiplies the jets' gradient as it is read: the same as
+    # scaling it beforehand, bit for bit (1 leaves it unchanged)
+    for sv in (1.0, -2.75):
+        s = torch.full((), sv, device="cuda")
+        gseg_s = torch.empty_like(plan.gseg)
+        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC, gseg_s, B, s)
+        gseg_r = torch.empty_like(plan.gseg)
+        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC * sv, gseg_r, B)
+        assert torch.equal(gseg_s, gseg_r), sv

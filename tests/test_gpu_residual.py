@@ -12,7 +12,11 @@ transpose, ops.ResidualLoss) against the problems' own loss_fn.
   two terms, a column without gradient) against the float64 formula;
 * the per-point gather of a trainable vector and its transpose
   (fbpx_segment_sum) against float64 index_add, with unsorted indices and
-  empty segments.
+  empty segments;
+* fbpx_residual_loss_grad (the loss launch that also writes the column
+  gradients at unit upstream gradient) against fbpx_residual_loss followed
+  by fbpx_residual_loss_backward with g_loss = 1, bit for bit, and
+  fbpx_segment_sum's scale against scaling afterwards.
 """
 
 import pytest
@@ -264,3 +268,81 @@ def test_gathered_column_and_accumulate(n, n_vec, use_done):
     assert ((gv.double().cpu() - gref).abs().max() / gref.abs().max()).item() <= 1e-4
     if use_done:
         assert int(done.item()) == 0
+
+
+@pytest.mark.parametrize("n,n_vec,use_done", [(1000, 7, True), (300_000, 256, False), (300_000, 256, True)])
+def test_loss_grad_launch_equals_loss_then_transpose(n, n_vec, use_done):
+    """fbpx_residual_loss_grad: r and the loss equal fbpx_residual_loss's and
+    the column gradients (two plain columns and a gathered one) equal
+    fbpx_residual_loss_backward's at g_loss = 1, bit for bit; the gathered
+    column's per-point gradient summed by fbpx_segment_sum with scale s
+    equals the plain sum times s, bit for bit, and the transpose at g_loss = s
+    within 1e-6 relative."""
+    import ctypes
+    from fbpinns_amd import _lib, ops
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(int(n) % 89)
+    a = torch.randn(n, generator=g).to(dev)
+    b = torch.randn(n, generator=g).to(dev)
+    v = torch.randn(n_vec, generator=g).to(dev)
+    idx = torch.randint(0, n_vec, (n,), generator=g).to(dev, torch.int32)
+    coef = torch.tensor([1.0, 2.0, -0.5, 0.75], device=dev)
+    # r = a + 2 v[idx] b - 0.5 + 0.75 a^2
+    fa, fb = [0, 2, -1, 0], [-1, 1, -1, 0]
+    cols = [a, b, v]
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = _lib.lib()
+    ptrs = (ctypes.c_void_p * 3)(*[c.data_ptr() for c in cols])
+    iptrs = (ctypes.c_void_p * 3)(None, None, idx.data_ptr())
+    cfa, cfb = (ctypes.c_int32 * 4)(*fa), (ctypes.c_int32 * 4)(*fb)
+    cp = ctypes.c_void_p(coef.data_ptr())
+    done = torch.zeros(1, dtype=torch.int32, device=dev)
+    dptr = ctypes.c_void_p(done.data_ptr()) if use_done else None
+    outs = {}
+    for form in ("grad", "loss"):
+        r = torch.empty(n, device=dev)
+        scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * 4, device=dev)
+        loss = torch.full((), 0.25, device=dev)
+        gr = [torch.full((n,), float("nan"), device=dev) for _ in range(3)]
+        gptrs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in gr])
+        args = (stream, n, 3, ptrs, iptrs, 4, cfa, cfb, cp, 0.6, ctypes.c_void_p(r.data_ptr()),
+                ctypes.c_void_p(scratch.data_ptr()), ctypes.c_void_p(loss.data_ptr()), 1, dptr)
+        if form == "grad":
+            assert L.fbpx_residual_loss_grad(*args, gptrs) == 0
+        else:
+            assert L.fbpx_residual_loss(*args) == 0
+            one = torch.ones(1, device=dev)
+            assert L.fbpx_residual_loss_backward(stream, n, 3, ptrs, iptrs, 4, cfa, cfb, cp,
+                                                 ctypes.c_void_p(r.data_ptr()), 0.6,
+                                                 ctypes.c_void_p(one.data_ptr()), gptrs,
+                                                 ctypes.c_void_p(scratch.data_ptr()), None, dptr) == 0
+        torch.cuda.synchronize()
+        outs[form] = (r, loss, gr)
+        if use_done:
+            assert int(done.item()) == 0
+    (r1, l1, g1), (r0, l0, g0) = outs["grad"], outs["loss"]
+    print(f"[loss grad] n={n} loss {l1.item():.9e} / {l0.item():.9e}", flush=True)
+    assert torch.equal(r1, r0) and torch.equal(l1, l0)
+    for f in range(3):
+        assert torch.equal(g1[f], g0[f]), f
+    # the gathered column's transpose with a scale
+    perm = torch.argsort(idx.long(), stable=True).to(torch.int32)
+    seg = torch.zeros(n_vec + 1, dtype=torch.int64, device=dev)
+    seg[1:] = torch.cumsum(torch.bincount(idx.long(), minlength=n_vec), 0)
+    seg = seg.to(torch.int32)
+    s = torch.full((), -1.3, device=dev)
+    plain, scaled = torch.empty(n_vec, device=dev), torch.empty(n_vec, device=dev)
+    ops.segment_sum(seg, perm, g1[2], plain)
+    ops.segment_sum(seg, perm, g1[2], scaled, s)
+    assert torch.equal(scaled, plain * -1.3)
+    gs = torch.empty(n, device=dev)
+    scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * 4, device=dev)
+    sp = torch.full((1,), -1.3, device=dev)
+    gptrs = (ctypes.c_void_p * 3)(None, None, gs.data_ptr())
+    assert L.fbpx_residual_loss_backward(stream, n, 3, ptrs, iptrs, 4, cfa, cfb, cp, ctypes.c_void_p(r0.data_ptr()),
+                                         0.6, ctypes.c_void_p(sp.data_ptr()), gptrs,
+                                         ctypes.c_void_p(scratch.data_ptr()), None, dptr) == 0
+    ref = torch.empty(n_vec, device=dev)
+    ops.segment_sum(seg, perm, gs, ref)
+    torch.cuda.synchronize()
+    assert ((scaled - ref).abs().max() / ref.abs().max()).item() < 1e-6

@@ -393,3 +393,59 @@ def test_adam_rows_partial_equals_reduce_then_adam():
     print(f"[adam partial] rows={plan.rows.numel()} P={eng.net.P}", flush=True)
     for a, b in zip(*out):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("name,unit_seed_path", [("burgers_small", True), ("wave_small", True),
+                                                 ("ho_hard_cfg1", True), ("ho_inverse_mu", False)])
+def test_unit_seed_gradients_across_interleaved_forwards(name, unit_seed_path, monkeypatch):
+    """The fused loss's forward writes the loss's jet gradients at unit
+    upstream gradient (fbpx_residual_loss_grad) into a buffer its backward
+    reads, unless a residual coefficient is trained (the oscillator's mu).
+    Two forwards before their backwards (the second takes the buffer over,
+    so the first backward runs the residual transposes instead) give the
+    gradients of separate forward/backward pairs bit for bit, and an
+    upstream gradient of 2.5 gives 2.5 times them within 1e-6 (1e-5 for a
+    trained coefficient's gradient, one float32 sum over the points with
+    cancellation, where the seed enters every term before the sum)."""
+    from fbpinns_amd import ops
+    eng, *_ = _build(name)
+    plan, ep = eng.plan, eng.epilogue
+    plan.defer_rows = False
+    calls = []
+    transpose = ops.residual_loss_backward
+    monkeypatch.setattr(ops, "residual_loss_backward", lambda *a: (calls.append(1), transpose(*a))[1])
+
+    def forward(scale):
+        params, pleaf = eng._params_for_step()
+        th = (eng.theta.detach() * scale).requires_grad_(True)
+        return ep.jets_loss(th, plan, params), [th] + ([pleaf] if pleaf is not None else [])
+
+    def grads(loss, leaves, seed=None):
+        out = torch.autograd.grad(loss, leaves, grad_outputs=seed, allow_unused=True)
+        return [None if g is None else g.detach().clone() for g in out]
+
+    la, va = forward(1.0)
+    ga = grads(la, va)
+    lb, vb = forward(1.01)
+    gb = grads(lb, vb)
+    n_res = len(ep.fused)
+    assert len(calls) == (0 if unit_seed_path else 2 * n_res)
+    del calls[:]
+    l0, v0 = forward(1.0)
+    l1, v1 = forward(1.01)
+    g1 = grads(l1, v1)
+    g0 = grads(l0, v0)
+    # the first forward's buffer was taken over: its backward ran the transposes
+    assert len(calls) == (1 if unit_seed_path else 2) * n_res
+    print(f"[unit seed] {name} residuals={n_res} loss {l0.item():.8e} / {l1.item():.8e}", flush=True)
+    assert torch.equal(l0, la) and torch.equal(l1, lb)
+    for x, y in list(zip(g0, ga)) + list(zip(g1, gb)):
+        assert (x is None) == (y is None)
+        assert x is None or torch.equal(x, y)
+    l2, v2 = forward(1.0)
+    g2 = grads(l2, v2, torch.full((), 2.5, device=l2.device))
+    for j, (x, y) in enumerate(zip(g2, ga)):
+        if x is not None:
+            e = ((x - 2.5 * y).abs().max() / (2.5 * y).abs().max()).item()
+            print(f"  seed 2.5: grad {tuple(x.shape)} rel {e:.1e}", flush=True)
+            assert e <= (1e-6 if j == 0 else 1e-5)
