@@ -664,6 +664,12 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
     if constexpr (GRAD) residual_col_grads<NT>(a, cf, xa, xb, sr, k0, stride, n);
   }
   const float tot = block_tree_sum(s, sm);
+  if (done && gridDim.x == 1) {
+    // the only workgroup: its total is the sum that last_block_sum would
+    // form (the other threads add zeros), with no fence, atomic or scratch
+    if (threadIdx.x == 0) loss[0] = acc ? loss[0] + scale * tot : scale * tot;
+    return;
+  }
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
   if (done) last_block_sum(scratch, 1, scale, loss, acc, done, sm);
 }
@@ -708,13 +714,20 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
     }
     residual_col_grads<NT>(a, cf, xa, xb, sr, k0, stride, n);
   }
+  // one workgroup: its totals are the sums (as in residual_kernel)
+  const bool only = done && gridDim.x == 1;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const float tot = block_tree_sum(acc[t], sm);
-    if (threadIdx.x == 0) scratch[blockIdx.x * NT + t] = tot;
+    if (threadIdx.x == 0) {
+      if (only)
+        gcoef[t] = tot;
+      else
+        scratch[blockIdx.x * NT + t] = tot;
+    }
     __syncthreads();
   }
-  if (done) last_block_sum(scratch, NT, 1.f, gcoef, false, done, sm);
+  if (done && !only) last_block_sum(scratch, NT, 1.f, gcoef, false, done, sm);
 }
 
 // out[j] = scale * sum over b < nb of scratch[b * ncol + j] (+ out[j] on

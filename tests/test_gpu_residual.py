@@ -182,15 +182,15 @@ def test_gather_transpose_matches_index_add(n, n_vec):
     assert (gv[n_vec - 2:].cpu() == 0).all() if n_vec > 2 else True
 
 
-def test_single_launch_sum_equals_two_launch_sum():
+@pytest.mark.parametrize("n", [200_000, 1000])
+def test_single_launch_sum_equals_two_launch_sum(n):
     """fbpx_residual_loss with a workgroup counter (the last workgroup sums
     the partials) and without one (a second launch sums them): the same
     fixed-order sum, bit for bit, over 196 workgroups (the single-launch
-    form is taken up to 256)."""
+    form is taken up to 256) and over one (whose total is the sum)."""
     import ctypes
     from fbpinns_amd import _lib
     dev = torch.device("cuda")
-    n = 200_000
     g = torch.Generator().manual_seed(5)
     cols = [torch.randn(n, generator=g).to(dev) for _ in range(2)]
     coef = torch.tensor([1.0, 0.5], device=dev)
