@@ -999,9 +999,11 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
   for (int j = 0; j < O::NOB; ++j) dW2[j] = f4{0.f, 0.f, 0.f, 0.f};
   float dbl = 0.f;
 
-  // every wave runs the same number of tiles (barriers below); lanes past the
-  // chunk end carry zero upstream gradient and so contribute nothing
-  const int ntile = (end - beg + 63) >> 6;
+  // with shared dW2 blocks every wave runs the same number of tiles (barriers
+  // below); with wave-private blocks (H <= 32) a wave runs only the tiles in
+  // which its 16-pair sub-tile holds a pair of the chunk.  Lanes past the
+  // chunk end carry zero upstream gradient and so contribute nothing.
+  const int ntile = O::PRIV ? (end - beg - 16 * wave + 63) >> 6 : (end - beg + 63) >> 6;
   // each tile's point and upstream jets are fetched during the previous tile
   // (indices at the start of its column-0 phase, data before its last dW2
   // column), so the two dependent global loads do not stall the tile start
