@@ -16,6 +16,8 @@ FBPX_CHUNK = 512
 FBPX_MAX_REQ = 16
 FBPX_MAX_TERMS = 8
 FBPX_RESIDUAL_BLOCKS = 1024
+FBPX_POU_LOSS_RES = 4
+FBPX_POU_LOSS_COLS = 8
 
 _c_i32 = ctypes.c_int32
 _c_p = ctypes.c_void_p
@@ -52,6 +54,14 @@ class fbpx_takes_info(ctypes.Structure):
                 ("n_chunks", _c_i32), ("n_fbins", _c_i32), ("n_bbins", _c_i32), ("npou", _c_i32)]
 
 
+class fbpx_residual(ctypes.Structure):
+    _fields_ = [("offset", _c_i32), ("n", _c_i32), ("n_cols", _c_i32), ("n_terms", _c_i32),
+                ("weight", ctypes.c_float), ("coef", _c_p), ("fa", _c_i32 * FBPX_MAX_TERMS),
+                ("fb", _c_i32 * FBPX_MAX_TERMS), ("jet", _c_i32 * FBPX_POU_LOSS_COLS),
+                ("col", _c_p * FBPX_POU_LOSS_COLS), ("col_idx", _c_p * FBPX_POU_LOSS_COLS),
+                ("gcol", _c_p * FBPX_POU_LOSS_COLS), ("r", _c_p)]
+
+
 # name -> (restype, argtypes); mirrors include/fbpx.h
 SIGNATURES = {
     "fbpx_inside_count": (_c_i32, [_c_p, _c_p, _c_i32, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p]),
@@ -73,7 +83,7 @@ SIGNATURES = {
                                    _c_p, _c_p, _c_p]),
     "fbpx_occupancy": (_c_i32, [ctypes.POINTER(fbpx_net), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32)]),
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
-                               _c_p, _c_p, _c_p]),
+                               _c_p, _c_p, _c_p, _c_p]),
     "fbpx_reduce_rows": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p]),
     "fbpx_adam_rows": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p,
                                 _c_d, _c_d, _c_d, _c_d]),
@@ -92,6 +102,9 @@ SIGNATURES = {
     "fbpx_residual_loss_grad": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p),
                                          _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p,
                                          ctypes.c_float, _c_p, _c_p, _c_p, _c_i32, _c_p, ctypes.POINTER(_c_p)]),
+    "fbpx_pou_loss": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p,
+                               _c_p, _c_i32, ctypes.POINTER(fbpx_residual), _c_p, _c_p, _c_p, ctypes.c_int64, _c_p,
+                               _c_i32, _c_p]),
     "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p),
                                              ctypes.POINTER(_c_p), _c_i32, ctypes.POINTER(_c_i32),
                                              ctypes.POINTER(_c_i32), _c_p, _c_p, ctypes.c_float, _c_p,

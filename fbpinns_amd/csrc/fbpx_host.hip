@@ -89,7 +89,8 @@ int fbpx_forward(void* stream, const fbpx_net* net, const float* theta, const fl
 }
 
 int fbpx_backward(void* stream, const fbpx_net* net, const float* theta, const float* dom, const float* x,
-                  const fbpx_worklist* wl, const uint8_t* chunk_train, const float* gseg, float* partial) {
+                  const fbpx_worklist* wl, const uint8_t* chunk_train, const float* gseg, const float* g_scale,
+                  float* partial) {
   int rc = check_pair_args(net, theta, dom, x, wl, partial);
   if (rc) return rc;
   if (wl->n_chunks > 0 && (!gseg || !wl->pair_seg)) {
@@ -97,7 +98,7 @@ int fbpx_backward(void* stream, const fbpx_net* net, const float* theta, const f
     return FBPX_EINVAL;
   }
   static const launch_fn fns[3] = {launch_bwd_xd1, launch_bwd_xd2, launch_bwd_xd3};
-  NetArgs a{theta, dom, x, *wl, chunk_train, gseg, partial};
+  NetArgs a{theta, dom, x, *wl, chunk_train, gseg, partial, g_scale};
   return dispatch(fns, net, a, (hipStream_t)stream);
 }
 

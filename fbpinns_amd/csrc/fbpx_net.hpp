@@ -643,7 +643,8 @@ __device__ inline void pair_upstream(const float* gU, const float* xv, const Dom
 // gradients (ssmall[wave][kind][u]; kinds WL, B1, W1[:, i], B2) and the
 // per-lane output-bias partials (sbl) into sacc, then writes the row.
 template <int XD, int H, int NH>
-__device__ inline void finish_row(float* sacc, const float* ssmall, const float* sbl, int nw, float* out) {
+__device__ inline void finish_row(float* sacc, const float* ssmall, const float* sbl, int nw, float* out,
+                                  float gs) {
   using L = Layout<XD, H, NH>;
   constexpr int NS = (NH == 2 ? 3 : 2) + XD;
   constexpr int K_WL = 0, K_B1 = 1, K_W1 = 2;
@@ -664,13 +665,15 @@ __device__ inline void finish_row(float* sacc, const float* ssmall, const float*
     sacc[L::BL] = s;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < L::P; i += blockDim.x) out[i] = sacc[i];
+  // gs: d L / d loss for a gseg at unit upstream gradient (the row is linear
+  // in gseg; 1 leaves it unchanged)
+  for (int i = threadIdx.x; i < L::P; i += blockDim.x) out[i] = sacc[i] * gs;
 }
 
 // [xd, h, 1]: no hidden-to-hidden layer; waves work independently.
 template <int XD, int H, int D1M, int D2M>
 __device__ inline void bwd_one(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
-                               const float* gseg, float* partial, int chunk) {
+                               const float* gseg, float* partial, int chunk, float gs) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 1>;
   using SM = Small<XD, H, 1>;
@@ -776,7 +779,7 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   }
   if (g == 0) sbl[wave * 16 + pl] = dbl;
   __syncthreads();
-  finish_row<XD, H, 1>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P);
+  finish_row<XD, H, 1>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
 }
 
 // W2^T G for one sub-tile: GA[ib] (A operand W2^T[i = 16ib+pl][o = 16ob+4g+ks]
@@ -916,7 +919,7 @@ __device__ inline void store_tile(float* tl, const f4* v) {
 // [xd, h, h, 1]: hidden-to-hidden layer on MFMA, streamed one axis at a time.
 template <int XD, int H, int D1M, int D2M>
 __device__ inline void bwd_two(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
-                               const float* gseg, float* partial, int chunk) {
+                               const float* gseg, float* partial, int chunk, float gs) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 2>;
   using SM = Small<XD, H, 2>;
@@ -1307,23 +1310,27 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __syncthreads();
   }
-  finish_row<XD, H, 2>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P);
+  finish_row<XD, H, 2>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
 }
 
+// g_scale: NULL, or d L / d loss when gseg holds the segment gradients at
+// unit upstream gradient; it multiplies each chunk's gradient row as the row
+// is written (finish_row).
 template <int XD, int H, int NH, int D1M, int D2M>
 __global__ void __launch_bounds__(256, 2)
 bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, const uint8_t* __restrict__ chunk_train, const float* __restrict__ gseg,
-           float* __restrict__ partial) {
+           const float* __restrict__ g_scale, float* __restrict__ partial) {
+  const float gs = g_scale ? g_scale[0] : 1.f;
   // persistent: workgroup b runs the chunks of backward bin b in order
   const int c0 = wl.bbin_chunk_start[blockIdx.x], c1 = wl.bbin_chunk_start[blockIdx.x + 1];
   for (int chunk = c0; chunk < c1; ++chunk) {
     if (chunk_train != nullptr && chunk_train[chunk] == 0) continue;
     __syncthreads();   // every wave is done with the previous chunk's LDS
     if constexpr (NH == 1)
-      bwd_one<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
+      bwd_one<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs);
     else
-      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk);
+      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs);
   }
 }
 

@@ -201,14 +201,15 @@ __device__ inline void leibniz_T(const float* Bj, const float* gc, float* gq) {
   }
 }
 
+// The POU-combined jets of point p (its segments' sums of u*w jets divided by
+// their window sums with the quotient rule, averaged over the levels,
+// trainers.py:162-170) and, with A != NULL, the affine constraining_fn
+// applied to them (trainers.py:174).
 template <class S>
-__global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
-                                                         const float* __restrict__ wsum, const float* __restrict__ A,
-                                                         const float* __restrict__ B, float* __restrict__ ujet) {
+__device__ inline void pou_point(const fbpx_worklist& wl, const float* __restrict__ pairbuf,
+                                 const float* __restrict__ wsum, const float* __restrict__ A,
+                                 const float* __restrict__ B, int p, float* acc) {
   constexpr int J = S::J;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= wl.n_points) return;
-  float acc[J];
 #pragma unroll
   for (int c = 0; c < J; ++c) acc[c] = 0.f;
   for (int seg = wl.point_seg_start[p]; seg < wl.point_seg_start[p + 1]; ++seg) {
@@ -240,26 +241,18 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
 #pragma unroll
     for (int c = 0; c < J; ++c) acc[c] = out[c];
   }
-#pragma unroll
-  for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = acc[c];
 }
 
-// g_scale != NULL: g_ujet is scaled by g_scale[0] as it is read (d L / d
-// loss applied to jets computed at unit upstream gradient).
+// Its transpose at point p: from d loss / d (the point's jets) gq to the
+// rows of gseg of the point's segments.
 template <class S>
-__global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
-                                                           const float* __restrict__ B,
-                                                           const float* __restrict__ g_ujet,
-                                                           const float* __restrict__ g_scale,
-                                                           float* __restrict__ gseg) {
+__device__ inline void pou_point_T(const fbpx_worklist& wl, const float* __restrict__ wsum,
+                                   const float* __restrict__ B, const float* g, int p, float* __restrict__ gseg) {
   constexpr int J = S::J;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= wl.n_points) return;
   const float inv = 1.f / (float)wl.npou;
-  const float sc = g_scale ? g_scale[0] : 1.f;
   float gq[J];
 #pragma unroll
-  for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p] * sc;
+  for (int c = 0; c < J; ++c) gq[c] = g[c];
   if (B != nullptr) {
     float Bj[J], gc[J];
 #pragma unroll
@@ -279,6 +272,37 @@ __global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, con
 #pragma unroll
     for (int c = 0; c < J; ++c) gseg[(size_t)seg * J + c] = gU[c];
   }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
+                                                         const float* __restrict__ wsum, const float* __restrict__ A,
+                                                         const float* __restrict__ B, float* __restrict__ ujet) {
+  constexpr int J = S::J;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wl.n_points) return;
+  float acc[J];
+  pou_point<S>(wl, pairbuf, wsum, A, B, p, acc);
+#pragma unroll
+  for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = acc[c];
+}
+
+// g_scale != NULL: g_ujet is scaled by g_scale[0] as it is read (d L / d
+// loss applied to jets computed at unit upstream gradient).
+template <class S>
+__global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
+                                                           const float* __restrict__ B,
+                                                           const float* __restrict__ g_ujet,
+                                                           const float* __restrict__ g_scale,
+                                                           float* __restrict__ gseg) {
+  constexpr int J = S::J;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= wl.n_points) return;
+  const float sc = g_scale ? g_scale[0] : 1.f;
+  float gq[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p] * sc;
+  pou_point_T<S>(wl, wsum, B, gq, p, gseg);
 }
 
 // ------------------------------------------------------------------ affine constraining function
@@ -761,6 +785,168 @@ __global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restr
   if (threadIdx.x == 0) out[s] = scale ? t * scale[0] : t;
 }
 
+// ------------------------------------------------------------------ POU reduction + residual losses + transpose
+
+// v[c] for a runtime c < N (selects; a register array indexed at run time
+// would live in scratch memory).
+template <int N>
+__device__ inline float sel(const float* v, int c) {
+  float r = v[0];
+#pragma unroll
+  for (int k = 1; k < N; ++k) r = (c == k) ? v[k] : r;
+  return r;
+}
+
+// The residuals of one fused launch with their host-computed scales, the
+// same float expressions as the separate launches use.
+struct PouLossArgs {
+  int n_res;
+  fbpx_residual res[FBPX_POU_LOSS_RES];
+  float scale[FBPX_POU_LOSS_RES];   // weight / n
+  float w2n[FBPX_POU_LOSS_RES];     // 2 weight / n
+};
+
+// loss = sum over residuals i of scale_i * (sum over the nb workgroups b of
+// part[b * n_res + i]) (the first added to loss[0] on entry when acc), by one
+// workgroup in a fixed order, as the separate launches accumulate them.
+__device__ inline void pou_loss_total(const float* part, int nb, const PouLossArgs& a, float* loss, bool acc,
+                                      float* sm) {
+  float total = 0.f;   // thread 0's
+  for (int i = 0; i < a.n_res; ++i) {
+    float v = 0.f;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b * a.n_res + i];
+    const float t = block_tree_sum(v, sm);
+    if (threadIdx.x == 0) {
+      const float term = a.scale[i] * t;
+      total = i > 0 ? total + term : acc ? loss[0] + term : term;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = total;
+}
+
+// One thread per point: pou_point (ujet written), every residual whose
+// points hold it (r, the loss's squares, d loss / d column at unit upstream
+// gradient with the arithmetic of residual_kernel<NT, true>: jet columns
+// into the point's jet gradient, memory columns into gcol), then pou_point_T
+// of the jet gradient into gseg.  part[block * n_res + i]: the workgroup's
+// sum of r^2 of residual i; done != NULL: the last workgroup (or the only
+// one, directly) forms the loss.
+template <class S>
+__global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
+                                                       const float* __restrict__ wsum, const float* __restrict__ A,
+                                                       const float* __restrict__ B, PouLossArgs a,
+                                                       float* __restrict__ ujet, float* __restrict__ gseg,
+                                                       float* part, float* loss, bool acc, int32_t* done) {
+  constexpr int J = S::J, NC = FBPX_POU_LOSS_COLS, NT = FBPX_MAX_TERMS, NR = FBPX_POU_LOSS_RES;
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  float sq[NR];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) sq[i] = 0.f;
+  if (p < wl.n_points) {
+    float u[J], g[J];
+    pou_point<S>(wl, pairbuf, wsum, A, B, p, u);
+#pragma unroll
+    for (int c = 0; c < J; ++c) {
+      ujet[(size_t)c * wl.n_points + p] = u[c];
+      g[c] = 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      if (i >= a.n_res) break;
+      const fbpx_residual& R = a.res[i];
+      const int k = p - R.offset;
+      if (k < 0 || k >= R.n) continue;
+      float F[NC];
+#pragma unroll
+      for (int f = 0; f < NC; ++f) {
+        F[f] = 0.f;
+        if (f < R.n_cols)
+          F[f] = R.jet[f] >= 0 ? sel<J>(u, R.jet[f]) : R.col[f][R.col_idx[f] ? R.col_idx[f][k] : k];
+      }
+      float xa[NT], xb[NT], cf[NT];
+      float r = 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        xa[t] = xb[t] = cf[t] = 0.f;
+        if (t < R.n_terms) {
+          cf[t] = R.coef[t];
+          xa[t] = R.fa[t] < 0 ? 1.f : sel<NC>(F, R.fa[t]);
+          xb[t] = R.fb[t] < 0 ? 1.f : sel<NC>(F, R.fb[t]);
+          r = fmaf(cf[t], xa[t] * xb[t], r);
+        }
+      }
+      R.r[k] = r;
+      sq[i] = fmaf(r, r, 0.f);
+      const float sr = a.w2n[i] * r;
+#pragma unroll
+      for (int f = 0; f < NC; ++f) {
+        if (f >= R.n_cols) break;
+        float gf = 0.f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          if (t >= R.n_terms) break;
+          if (R.fa[t] == f) gf = fmaf(cf[t], xb[t], gf);
+          if (R.fb[t] == f) gf = fmaf(cf[t], xa[t], gf);
+        }
+        const float gv = sr * gf;
+        if (R.jet[f] >= 0) {
+#pragma unroll
+          for (int c = 0; c < J; ++c)
+            if (c == R.jet[f]) g[c] = gv;
+        } else if (R.gcol[f] != nullptr) {
+          R.gcol[f][k] = gv;
+        }
+      }
+    }
+    pou_point_T<S>(wl, wsum, B, g, p, gseg);
+  }
+  // per-residual workgroup sums of r^2
+  float tot[NR];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    tot[i] = 0.f;
+    if (i < a.n_res) tot[i] = block_tree_sum(sq[i], sm);
+    __syncthreads();
+  }
+  if (done && gridDim.x == 1) {
+    // the only workgroup: its sums are the totals (no fence, counter or scratch)
+    if (threadIdx.x == 0) {
+      float total = 0.f;
+      for (int i = 0; i < a.n_res; ++i) {
+        const float term = a.scale[i] * tot[i];
+        total = i > 0 ? total + term : acc ? loss[0] + term : term;
+      }
+      loss[0] = total;
+    }
+    return;
+  }
+  if (threadIdx.x == 0)
+    for (int i = 0; i < a.n_res; ++i) part[blockIdx.x * a.n_res + i] = tot[i];
+  if (done) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();   // this workgroup's partials are visible before it is counted
+      last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();     // every other workgroup's partials are visible here
+    pou_loss_total(part, gridDim.x, a, loss, acc, sm);
+    if (threadIdx.x == 0) atomicExch(done, 0);
+  }
+}
+
+// The second launch of pou_loss_kernel's sum when its grid is too large for
+// a last-workgroup sum.
+__global__ void __launch_bounds__(256) pou_loss_sum_kernel(const float* part, int nb, PouLossArgs a, float* loss,
+                                                           bool acc) {
+  __shared__ float sm[4];
+  pou_loss_total(part, nb, a, loss, acc, sm);
+}
+
 __global__ void count_increment_kernel(int32_t* count) {
   if (threadIdx.x == 0) count[0] = count[0] + 1;
 }
@@ -1178,6 +1364,72 @@ int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const f
   if (gcoef && !one)
     hipLaunchKernelGGL(sum_columns_kernel, dim3(1), dim3(256), 0, s, scratch, nb, n_terms, 1.f, gcoef, false);
   return hip_check(hipGetLastError(), "residual_loss_backward");
+}
+
+int fbpx_pou_loss(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* pairbuf,
+                  const float* wsum, const float* A, const float* B, int32_t n_res, const fbpx_residual* res,
+                  float* ujet, float* gseg, float* scratch, int64_t scratch_len, float* loss, int32_t accumulate,
+                  int32_t* done) {
+  int J, rc;
+  if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  if ((A == nullptr) != (B == nullptr)) {
+    set_error("pou_loss: A and B are both given or both NULL");
+    return FBPX_EINVAL;
+  }
+  if (n_res < 1 || n_res > FBPX_POU_LOSS_RES || !res || !pairbuf || !wsum || !ujet || !gseg || !scratch || !loss) {
+    set_error("pou_loss: n_res=%d (1 .. %d) or a null pointer", n_res, FBPX_POU_LOSS_RES);
+    return FBPX_EINVAL;
+  }
+  const int64_t nb = ((int64_t)wl->n_points + 255) / 256;
+  if (wl->n_points < 1 || scratch_len < nb * n_res) {
+    set_error("pou_loss: %d points, scratch of %lld floats (need %lld)", wl->n_points, (long long)scratch_len,
+              (long long)(nb * n_res));
+    return FBPX_EINVAL;
+  }
+  PouLossArgs a{};
+  a.n_res = n_res;
+  for (int i = 0; i < n_res; ++i) {
+    const fbpx_residual& R = res[i];
+    if (R.n < 1 || R.offset < 0 || (int64_t)R.offset + R.n > wl->n_points || R.n_cols < 1 ||
+        R.n_cols > FBPX_POU_LOSS_COLS || R.n_terms < 1 || R.n_terms > FBPX_MAX_TERMS || !R.coef || !R.r) {
+      set_error("pou_loss: residual %d: points [%d, %d + %d) of %d, %d columns (at most %d), %d terms (at most %d)",
+                i, R.offset, R.offset, R.n, wl->n_points, R.n_cols, FBPX_POU_LOSS_COLS, R.n_terms, FBPX_MAX_TERMS);
+      return FBPX_EINVAL;
+    }
+    for (int f = 0; f < R.n_cols; ++f) {
+      if (R.jet[f] >= J || (R.jet[f] < 0 && !R.col[f])) {
+        set_error("pou_loss: residual %d column %d: jet column %d of %d, memory column %p", i, f, R.jet[f], J,
+                  (const void*)R.col[f]);
+        return FBPX_EINVAL;
+      }
+    }
+    for (int t = 0; t < R.n_terms; ++t) {
+      if (R.fa[t] >= R.n_cols || R.fb[t] >= R.n_cols) {
+        set_error("pou_loss: residual %d term %d factors (%d, %d) outside %d columns", i, t, R.fa[t], R.fb[t],
+                  R.n_cols);
+        return FBPX_EINVAL;
+      }
+    }
+    a.res[i] = R;
+    for (int f = R.n_cols; f < FBPX_POU_LOSS_COLS; ++f) a.res[i].jet[f] = -1;
+    for (int t = 0; t < R.n_terms; ++t) {
+      a.res[i].fa[t] = R.fa[t] < 0 ? -1 : R.fa[t];
+      a.res[i].fb[t] = R.fb[t] < 0 ? -1 : R.fb[t];
+    }
+    a.scale[i] = R.weight / (float)R.n;        // as fbpx_residual_loss
+    a.w2n[i] = 2.f * R.weight / (float)R.n;    // as its transpose
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const bool acc = accumulate != 0;
+  int32_t* const one = nb <= kSingleLaunchBlocks ? done : nullptr;
+#define X(XD, D1, D2)                                                                                        \
+  if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                               \
+    hipLaunchKernelGGL((pou_loss_kernel<Schema<XD, D1, D2>>), dim3((unsigned)nb), dim3(256), 0, s, *wl, pairbuf, \
+                       wsum, A, B, a, ujet, gseg, scratch, loss, acc, one);
+  FBPX_SCHEMAS(X)
+#undef X
+  if (!one) hipLaunchKernelGGL(pou_loss_sum_kernel, dim3(1), dim3(256), 0, s, scratch, (int)nb, a, loss, acc);
+  return hip_check(hipGetLastError(), "pou_loss");
 }
 
 int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,

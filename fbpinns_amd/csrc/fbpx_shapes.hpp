@@ -28,6 +28,7 @@ struct NetArgs {
   const uint8_t* chunk_train;
   const float* gseg;
   float* out;  // pairbuf (forward) or partial (backward)
+  const float* g_scale = nullptr;  // backward: NULL or d L / d loss for unit-seed gseg
 };
 
 // Return FBPX_EUNSUPPORTED if the shape is not compiled, else launch status.

@@ -138,10 +138,15 @@ class Epilogue:
         # d loss / d (constrained jets): columns no residual reads stay 0
         self.gC = (torch.zeros((net.J, x.shape[0]), dtype=torch.float32, device=x.device)
                    if self.combined is not None else None)
-        # the fused loss's forward that last wrote gC at unit upstream
-        # gradient (its token), so that only its own backward reads it there
+        # the fused loss's forward that last wrote gC (or the plan's gseg) at
+        # unit upstream gradient (its token), so that only its own backward
+        # reads it there
         self._gC_owner = None
         self._grad_wanted = False
+        # fbpx_pou_loss takes every residual in one launch when they fit it
+        self.pou_loss_fits = self.combined is not None and len(self.fused) <= _lib.FBPX_POU_LOSS_RES and all(
+            len(keys) <= _lib.FBPX_POU_LOSS_COLS for _, _, keys, *_ in self.fused)
+        self._pou_loss_done = (torch.zeros(1, dtype=torch.int32, device=x.device) if self.pou_loss_fits else None)
 
     # -------------------------------------------------------------- set-up
 
@@ -473,9 +478,14 @@ class _JetsResidualLoss(torch.autograd.Function):
     the per-point gradients of gathered columns), and the backward starts at
     fbpx_pou_backward, scaled by the incoming gradient there and in
     fbpx_segment_sum: one launch per residual fewer, the same values bit for
-    bit at the step's unit seed.  The shared [J][n] buffer carries an owner
-    token; a backward that no longer owns it (another forward of this
-    epilogue ran in between) takes the transpose path above."""
+    bit at the step's unit seed.  When the residuals fit one fbpx_pou_loss
+    launch (at most FBPX_POU_LOSS_RES residuals of at most
+    FBPX_POU_LOSS_COLS columns) and the points one workgroup, that launch
+    does the POU reduction, every residual and the POU transpose into the
+    plan's gseg at unit upstream gradient, and the backward is fbpx_backward
+    with the incoming gradient as its g_scale.  The shared buffers (the [J][n] jet gradients, the plan's
+    gseg) carry an owner token; a backward that no longer owns them (another
+    forward of this epilogue ran in between) takes the transpose path above."""
 
     @staticmethod
     def _columns(ep, C, i, vecs):
@@ -503,36 +513,82 @@ class _JetsResidualLoss(torch.autograd.Function):
         ops.forward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
         A, B = ep.affine
         C = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=theta.device)
-        ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
         # the trainable vectors of the gathered columns (v[idx[k]] read in the
         # residual kernels), in key order
         pk = [data[k] for _, _, keys, data, *_ in ep.fused for k in keys if k[0] == "P"]
-        it = iter(pvecs)
         # every residual adds its mean into one loss scalar
         total = torch.empty((), dtype=torch.float32, device=theta.device)
-        rs = []
         # unit-seed gradients in the loss launch (see the class docstring)
         pregrad = ep._grad_wanted and ctx.needs_input_grad[0] and not any(ctx.needs_input_grad[4:4 + nr])
+        # one launch for the POU reduction, residuals and transpose pays off
+        # where its points fit one workgroup: a per-point chain of dependent
+        # loads that the separate launches overlap, and on larger grids a
+        # device-scope fence per workgroup (Burgers' first slab: 22.5 against
+        # 5.1 + 8.6 + 4.8 us; the oscillator's 200 points: one launch
+        # instead of three)
+        fused_pou = pregrad and ep.pou_loss_fits and wl.n_points <= 256
         gp = []
-        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
-            cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
-            r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=theta.device)
-            scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
-            if pregrad:
-                gidx, gcols = _JetsResidualLoss._gradient_columns(ep, i, gp)
-                ops.residual_loss_grad(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done,
-                                       gathered, idx, i > 0, gidx, gcols)
-            else:
-                ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done, gathered,
-                                  idx, i > 0)
-            rs.append(r)
+        if fused_pou:
+            rs = _JetsResidualLoss._pou_loss(ep, plan, C, coefs, pvecs, total, gp)
+        else:
+            ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
+            rs = []
+            it = iter(pvecs)
+            for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
+                cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
+                r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=theta.device)
+                scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=theta.device)
+                if pregrad:
+                    gidx, gcols = _JetsResidualLoss._gradient_columns(ep, i, gp)
+                    ops.residual_loss_grad(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done,
+                                           gathered, idx, i > 0, gidx, gcols)
+                else:
+                    ops.residual_loss(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total, done,
+                                      gathered, idx, i > 0)
+                rs.append(r)
         ctx.save_for_backward(theta, C, *coefs, *rs, *pvecs)
         ctx.plan, ctx.ep, ctx.nr, ctx.pk = plan, ep, nr, pk
         ctx.nvec = [v.numel() for v in pvecs]
         ctx.token = object() if pregrad else None
+        ctx.fused_pou = fused_pou
         ctx.gp = gp
         ep._gC_owner = ctx.token
         return total
+
+    @staticmethod
+    def _pou_loss(ep, plan, C, coefs, pvecs, total, gp):
+        """fbpx_pou_loss over every residual: C, the residuals (returned), the
+        gathered columns' per-point gradients (appended to gp) and plan.gseg
+        at unit upstream gradient, and the loss into total."""
+        net, wl = plan.net, plan.wl
+        it = iter(pvecs)
+        residuals, rs = [], []
+        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
+            off, n_c = ep.cranges[ic]
+            cols, idx, gcols = [], [], []
+            for c, k in zip(ep.combined[i], keys):
+                if c is not None:
+                    cols.append(int(c))
+                    idx.append(None)
+                    gcols.append(None)
+                elif k[0] == "P":
+                    cols.append(next(it))
+                    idx.append(data[k][1])
+                    gp.append(torch.empty(n_c, dtype=torch.float32, device=C.device))
+                    gcols.append(gp[-1])
+                else:
+                    cols.append(data[k])
+                    idx.append(None)
+                    gcols.append(None)
+            rs.append(torch.empty(n_c, dtype=torch.float32, device=C.device))
+            residuals.append(dict(offset=off, n=n_c, weight=weight, coef=coefs[i], fa=fa, fb=fb, cols=cols, idx=idx,
+                                  gcols=gcols, r=rs[-1]))
+        nb = (wl.n_points + 255) // 256
+        scratch = torch.empty(nb * len(residuals), dtype=torch.float32, device=C.device)
+        A, B = ep.affine
+        ops.pou_loss(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, A, B, residuals, C,
+                     plan.gseg, scratch, total, ep._pou_loss_done)
+        return rs
 
     @staticmethod
     def _gradient_columns(ep, i, gp):
@@ -559,8 +615,9 @@ class _JetsResidualLoss(torch.autograd.Function):
         coefs, rs, pvecs = rest[:nr], rest[nr:2 * nr], rest[2 * nr:]
         net, wl = plan.net, plan.wl
         g = g.contiguous()
-        # the forward's unit-seed gradients are still in ep.gC: scale by g
-        # where they are read; else the transposes write them here
+        # the forward's unit-seed gradients are still in ep.gC (or, from
+        # fbpx_pou_loss, in plan.gseg): scale by g where they are read; else
+        # the transposes write them here
         pregrad = ctx.token is not None and ep._gC_owner is ctx.token
         ep._gC_owner = None
         scale = g if pregrad else None
@@ -583,8 +640,12 @@ class _JetsResidualLoss(torch.autograd.Function):
             out = torch.empty(nv, dtype=torch.float32, device=g.device)
             ops.segment_sum(d[3], d[2], gcol, out, scale)
             gvecs.append(out)
-        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg, ep.affine[1],
-                         scale)
-        ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
-                     plan.gseg, plan.partial)
+        if pregrad and ctx.fused_pou:
+            ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                         plan.gseg, plan.partial, g)
+        else:
+            ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg,
+                             ep.affine[1], scale)
+            ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                         plan.gseg, plan.partial)
         return (ops.theta_gradient(plan, theta), None, None, None, *gcoefs, *gvecs)

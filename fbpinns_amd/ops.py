@@ -206,14 +206,78 @@ def pou_backward(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum
 
 @torch.library.custom_op("fbpx::backward", mutates_args=("partial",))
 def backward(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Tensor, wl_t: List[torch.Tensor],
-             wl_n: List[int], chunk_train: torch.Tensor, gseg: torch.Tensor, partial: torch.Tensor) -> None:
+             wl_n: List[int], chunk_train: torch.Tensor, gseg: torch.Tensor, partial: torch.Tensor,
+             g_scale: Optional[torch.Tensor] = None) -> None:
+    """g_scale: None, or a one-element tensor by which gseg is multiplied
+    (d L / d loss for a gseg written at unit upstream gradient)."""
     n = _net_from_list(net)
     w = _wl_struct(wl_t, wl_n)
     ct = _p(chunk_train, torch.uint8, "chunk_train") if chunk_train.numel() else None
+    if g_scale is not None and g_scale.numel() != 1:
+        raise FbpxError(f"backward: g_scale of {g_scale.numel()} values")
     check(lib().fbpx_backward(_stream(), ctypes.byref(n), _p(theta, torch.float32, "theta"),
                               _p(dom, torch.float32, "dom"), _p(x, torch.float32, "x"), ctypes.byref(w), ct,
-                              _p(gseg, torch.float32, "gseg"), _p(partial, torch.float32, "partial")),
+                              _p(gseg, torch.float32, "gseg"), _p(g_scale, torch.float32, "g_scale"),
+                              _p(partial, torch.float32, "partial")),
           "fbpx_backward")
+
+
+def pou_loss(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf: torch.Tensor, wsum: torch.Tensor,
+             A: Optional[torch.Tensor], B: Optional[torch.Tensor], residuals: list, ujet: torch.Tensor,
+             gseg: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
+             accumulate: bool = False) -> None:
+    """fbpx_pou_loss: POU reduction, affine constraint, the residual losses
+    and the POU transpose at unit upstream gradient in one launch.
+    residuals: dicts with offset, n, weight, coef (tensor), fa, fb (term
+    factor indices), cols (per column: an int jet column, or a tensor of
+    per-point data / a gathered vector), idx (per column: None or the int32
+    gather index), gcols (per column: None or a per-point gradient tensor),
+    r (the residual tensor).  The tensors stay referenced by the caller."""
+    if not 1 <= len(residuals) <= _lib.FBPX_POU_LOSS_RES:
+        raise FbpxError(f"pou_loss: {len(residuals)} residuals (at most {_lib.FBPX_POU_LOSS_RES})")
+    if (A is None) != (B is None) or (A is not None and (A.shape != ujet.shape or B.shape != ujet.shape)):
+        raise FbpxError("pou_loss: A and B are both given (shaped as ujet) or both None")
+    arr = (_lib.fbpx_residual * len(residuals))()
+    for d, R in zip(arr, residuals):
+        cols, fa, fb = R["cols"], R["fa"], R["fb"]
+        if not 1 <= len(cols) <= _lib.FBPX_POU_LOSS_COLS or not 1 <= len(fa) <= _lib.FBPX_MAX_TERMS \
+                or len(fb) != len(fa):
+            raise FbpxError(f"pou_loss: {len(cols)} columns (at most {_lib.FBPX_POU_LOSS_COLS}), {len(fa)}/{len(fb)} "
+                            f"term factors (at most {_lib.FBPX_MAX_TERMS})")
+        n = int(R["n"])
+        if R["r"].numel() != n or R["coef"].numel() != len(fa):
+            raise FbpxError(f"pou_loss: r of {R['r'].numel()} for {n} points, coef {R['coef'].numel()} for "
+                            f"{len(fa)} terms")
+        d.offset, d.n, d.n_cols, d.n_terms, d.weight = int(R["offset"]), n, len(cols), len(fa), float(R["weight"])
+        d.coef = _p(R["coef"], torch.float32, "coef")
+        d.r = _p(R["r"], torch.float32, "r")
+        for t, (a, b) in enumerate(zip(fa, fb)):
+            d.fa[t], d.fb[t] = int(a), int(b)
+        for f, c in enumerate(cols):
+            ix, gc = R["idx"][f], R["gcols"][f]
+            if isinstance(c, int):
+                d.jet[f] = c
+            else:
+                d.jet[f] = -1
+                d.col[f] = _p(c, torch.float32, "column")
+                if ix is None and c.numel() != n:
+                    raise FbpxError(f"pou_loss: column of {c.numel()} values for {n} points")
+            if ix is not None:
+                if ix.numel() != n:
+                    raise FbpxError(f"pou_loss: gather index of {ix.numel()} entries for {n} points")
+                d.col_idx[f] = _p(ix, torch.int32, "gather index")
+            if gc is not None:
+                if gc.numel() != n:
+                    raise FbpxError(f"pou_loss: gradient column of {gc.numel()} values for {n} points")
+                d.gcol[f] = _p(gc, torch.float32, "gradient column")
+    n_ = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    check(lib().fbpx_pou_loss(_stream(), ctypes.byref(n_), ctypes.byref(w), _p(pairbuf, torch.float32, "pairbuf"),
+                              _p(wsum, torch.float32, "wsum"), _p(A, torch.float32, "A"), _p(B, torch.float32, "B"),
+                              len(residuals), arr, _p(ujet, torch.float32, "ujet"), _p(gseg, torch.float32, "gseg"),
+                              _p(scratch, torch.float32, "scratch"), scratch.numel(), _p(loss, torch.float32, "loss"),
+                              int(bool(accumulate)), _p(done, torch.int32, "done")),
+          "fbpx_pou_loss")
 
 
 @torch.library.custom_op("fbpx::reduce_rows", mutates_args=("grad",))

@@ -223,10 +223,12 @@ int fbpx_pou_backward(void* stream, const fbpx_net* net, const fbpx_worklist* wl
  * activations.  One workgroup per backward bin; chunks outside the bins or
  * with chunk_train[c] == 0 (fixed subdomains, trainers.py:252-256 -- no
  * gradient) are skipped and their partial row is left untouched.
- * partial: [n_chunks][P] per-chunk gradient rows. */
+ * partial: [n_chunks][P] per-chunk gradient rows.  g_scale: NULL, or a
+ * DEVICE float by which gseg is multiplied (d L / d loss when gseg was
+ * written at unit upstream gradient by fbpx_pou_loss). */
 int fbpx_backward(void* stream, const fbpx_net* net, const float* theta,
                   const float* dom, const float* x, const fbpx_worklist* wl,
-                  const uint8_t* chunk_train, const float* gseg, float* partial);
+                  const uint8_t* chunk_train, const float* gseg, const float* g_scale, float* partial);
 
 /* Resident 256-thread workgroups per compute unit of fbpx_forward's and
  * fbpx_backward's kernel for this network and jet schema (register and LDS
@@ -365,6 +367,53 @@ int fbpx_residual_loss_grad(void* stream, int64_t n, int32_t n_cols, const float
                             const int32_t* const* col_idx, int32_t n_terms, const int32_t* fa, const int32_t* fb,
                             const float* coef, float weight, float* r, float* scratch, float* loss,
                             int32_t accumulate, int32_t* done, float* const* gcol);
+
+/*
+ * The POU reduction, the affine constraining function, the fused residual
+ * losses and the POU transpose at unit upstream gradient in one launch, one
+ * thread per point: fbpx_pou_reduce, then fbpx_residual_loss_grad for every
+ * residual, then fbpx_pou_backward of the jet gradients, with the same
+ * per-point arithmetic (ujet, r, the gathered columns' gradients and gseg
+ * equal theirs bit for bit; the loss sums the same squares in another fixed
+ * order).  Every (residual, jet column) pair reads a column of its own
+ * constraint, and no two residuals read the same jet column of a constraint
+ * (its gradient is written, not added).  The caller applies d L / d loss to
+ * gseg in fbpx_backward (g_scale) and to the gathered columns' gradients in
+ * fbpx_segment_sum (scale).
+ *
+ * Residual i covers the points [offset, offset + n) of the work list:
+ *   r[k] = sum over t < n_terms of coef[t] F(fa[t], k) F(fb[t], k)  (k < n),
+ *   F(f, k) = column jet[f] of point offset + k's constrained jets when
+ *             jet[f] >= 0, else col[f][k] (col[f][col_idx[f][k]] when
+ *             col_idx[f] is not NULL); 1 for f < 0,
+ *   loss += weight / n * sum over k of r[k]^2  (residuals in order; the
+ *           first adds to loss[0] on entry when accumulate != 0),
+ *   gcol[f][k] = 2 weight / n * r[k] * d r[k] / d F(f, k) for memory columns
+ *             (jet[f] < 0) with gcol[f] != NULL.
+ * ujet: [J][n_points] out (the constrained jets); gseg: [n_segs][J] out;
+ * scratch: ceil(n_points / 256) * n_res floats; done: as in
+ * fbpx_residual_loss (the last workgroup sums the partials when the grid has
+ * at most 256 workgroups, one workgroup writes them directly, else a second
+ * one-workgroup launch does).
+ */
+#define FBPX_POU_LOSS_RES 4
+#define FBPX_POU_LOSS_COLS 8
+typedef struct {
+  int32_t offset, n;                       /* points [offset, offset + n) */
+  int32_t n_cols, n_terms;                 /* <= FBPX_POU_LOSS_COLS, <= FBPX_MAX_TERMS */
+  float weight;
+  const float* coef;                       /* DEVICE, n_terms */
+  int32_t fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
+  int32_t jet[FBPX_POU_LOSS_COLS];         /* jet column, or -1: col[f] */
+  const float* col[FBPX_POU_LOSS_COLS];    /* DEVICE per-point data (n) or gathered vector */
+  const int32_t* col_idx[FBPX_POU_LOSS_COLS]; /* DEVICE gather index (n) or NULL */
+  float* gcol[FBPX_POU_LOSS_COLS];         /* DEVICE per-point gradient (n) or NULL */
+  float* r;                                /* DEVICE, n */
+} fbpx_residual;
+int fbpx_pou_loss(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* pairbuf,
+                  const float* wsum, const float* A, const float* B, int32_t n_res, const fbpx_residual* res,
+                  float* ujet, float* gseg, float* scratch, int64_t scratch_len, float* loss, int32_t accumulate,
+                  int32_t* done);
 
 /* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
  * pointer to d L / d loss):

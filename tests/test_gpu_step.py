@@ -449,3 +449,78 @@ def test_unit_seed_gradients_across_interleaved_forwards(name, unit_seed_path, m
             e = ((x - 2.5 * y).abs().max() / (2.5 * y).abs().max()).item()
             print(f"  seed 2.5: grad {tuple(x.shape)} rel {e:.1e}", flush=True)
             assert e <= (1e-6 if j == 0 else 1e-5)
+
+
+def _pou_loss_engine(name):
+    """The engine of a fused-loss case; batched_inverse has a gathered
+    trainable column (each instance's damping); poisson_full (262,144 points,
+    1,024 workgroups) sums the loss in a second launch."""
+    if name == "batched_inverse":
+        import test_gpu_inverse
+        return test_gpu_inverse._engine(4)[1]
+    if name == "poisson_full":
+        from fbpinns_amd import configs
+        from fbpinns_amd.trainers import FBPINNTrainer
+        eng, sched, *_ = FBPINNTrainer(configs.CONFIGS["poisson"](), write_outputs=False).setup(exact=False)
+        eng.set_active(next(iter(sched)))
+        return eng
+    return _build(name)[0]
+
+
+@pytest.mark.parametrize("name", ["ho_hard_cfg1", "ho_soft_two_constraints", "burgers_small", "wave_small",
+                                  "poisson_small", "batched_inverse", "poisson_full"])
+def test_pou_loss_launch_equals_separate_kernels(name):
+    """fbpx_pou_loss (POU reduction, affine constraint, every residual and the
+    POU transpose at unit upstream gradient in one launch) against
+    fbpx_pou_reduce, fbpx_residual_loss_grad per residual and
+    fbpx_pou_backward: the constrained jets, the residuals, the gathered
+    columns' gradients and the segment gradients bit for bit, the loss (the
+    same squares summed in another fixed order) within 1e-6."""
+    from fbpinns_amd import _lib, ops
+    from fbpinns_amd.epilogue import _JetsResidualLoss
+    eng = _pou_loss_engine(name)
+    plan, ep = eng.plan, eng.epilogue
+    # (the step takes this launch for work lists of at most 256 points; it is
+    # compared here on every case, larger grids included)
+    assert ep.combined is not None and ep.pou_loss_fits, name
+    net, wl = plan.net, plan.wl
+    params, _ = eng._params_for_step()
+    coefs = [ep._coef(static, dyn, params) for _, _, _, _, _, _, static, dyn, _ in ep.fused]
+    pvecs = [torch.as_tensor(data[k][0](params)).reshape(-1).to(torch.float32).contiguous()
+             for _, _, keys, data, *_ in ep.fused for k in keys if k[0] == "P"]
+    print(f"[pou loss] {name}: {len(ep.fused)} residuals, {wl.n_points} points, {len(pvecs)} gathered", flush=True)
+    ops.forward(net.as_list(), eng.theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.pairbuf)
+    A, B = ep.affine
+    dev = eng.theta.device
+    # separate kernels
+    C0 = torch.empty((net.J, wl.n_points), dtype=torch.float32, device=dev)
+    ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C0, A, B)
+    total0 = torch.empty((), dtype=torch.float32, device=dev)
+    rs0, gp0 = [], []
+    it = iter(pvecs)
+    for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
+        cols, gathered, idx = _JetsResidualLoss._columns(ep, C0, i, it)
+        r = torch.empty(ep.cranges[ic][1], dtype=torch.float32, device=dev)
+        scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=dev)
+        gidx, gcols = _JetsResidualLoss._gradient_columns(ep, i, gp0)
+        ops.residual_loss_grad(cols, list(fa), list(fb), coefs[i], weight, r, scratch, total0, done, gathered, idx,
+                               i > 0, gidx, gcols)
+        rs0.append(r)
+    gseg0 = torch.empty_like(plan.gseg)
+    ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, gseg0, B)
+    # one launch
+    C1 = torch.empty_like(C0)
+    total1 = torch.empty((), dtype=torch.float32, device=dev)
+    gp1 = []
+    plan.gseg.fill_(float("nan"))
+    rs1 = _JetsResidualLoss._pou_loss(ep, plan, C1, coefs, pvecs, total1, gp1)
+    torch.cuda.synchronize()
+    assert int(ep._pou_loss_done.item()) == 0
+    assert torch.equal(C1, C0)
+    assert len(rs1) == len(rs0) and all(torch.equal(a, b) for a, b in zip(rs1, rs0))
+    assert len(gp1) == len(gp0) and all(torch.equal(a, b) for a, b in zip(gp1, gp0))
+    ns = wl.n_segs
+    assert torch.equal(plan.gseg[:ns], gseg0[:ns])
+    e = abs(total1.item() - total0.item()) / abs(total0.item())
+    print(f"  loss {total1.item():.9e} / {total0.item():.9e} rel {e:.1e}", flush=True)
+    assert e <= 1e-6
