@@ -244,10 +244,12 @@ __device__ inline void pou_point(const fbpx_worklist& wl, const float* __restric
 }
 
 // Its transpose at point p: from d loss / d (the point's jets) gq to the
-// rows of gseg of the point's segments.
+// rows of gseg of the point's segments.  B: the affine map's [J][n_points]
+// jets, or with b_own the point's own J values.
 template <class S>
 __device__ inline void pou_point_T(const fbpx_worklist& wl, const float* __restrict__ wsum,
-                                   const float* __restrict__ B, const float* g, int p, float* __restrict__ gseg) {
+                                   const float* __restrict__ B, const float* g, int p, float* __restrict__ gseg,
+                                   bool b_own = false) {
   constexpr int J = S::J;
   const float inv = 1.f / (float)wl.npou;
   float gq[J];
@@ -257,7 +259,7 @@ __device__ inline void pou_point_T(const fbpx_worklist& wl, const float* __restr
     float Bj[J], gc[J];
 #pragma unroll
     for (int c = 0; c < J; ++c) {
-      Bj[c] = B[(size_t)c * wl.n_points + p];
+      Bj[c] = b_own ? B[c] : B[(size_t)c * wl.n_points + p];
       gc[c] = gq[c];
     }
     leibniz_T<S>(Bj, gc, gq);
@@ -829,9 +831,12 @@ __device__ inline void pou_loss_total(const float* part, int nb, const PouLossAr
 // points hold it (r, the loss's squares, d loss / d column at unit upstream
 // gradient with the arithmetic of residual_kernel<NT, true>: jet columns
 // into the point's jet gradient, memory columns into gcol), then pou_point_T
-// of the jet gradient into gseg.  part[block * n_res + i]: the workgroup's
-// sum of r^2 of residual i; done != NULL: the last workgroup (or the only
-// one, directly) forms the loss.
+// of the jet gradient into gseg.  The loads that do not depend on the POU
+// reduction (the affine jets, the residuals' memory columns) are issued
+// before it, so that a thread's chain of dependent loads is the reduction's
+// alone.  part[block * n_res + i]: the workgroup's sum of r^2 of residual i;
+// done != NULL: the last workgroup (or the only one, directly) forms the
+// loss.
 template <class S>
 __global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const float* __restrict__ pairbuf,
                                                        const float* __restrict__ wsum, const float* __restrict__ A,
@@ -845,8 +850,34 @@ __global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const f
 #pragma unroll
   for (int i = 0; i < NR; ++i) sq[i] = 0.f;
   if (p < wl.n_points) {
+    // independent of the reduction: the affine jets and the memory columns
+    float Aj[J], Bj[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) {
+      Aj[c] = A ? A[(size_t)c * wl.n_points + p] : 0.f;
+      Bj[c] = B ? B[(size_t)c * wl.n_points + p] : 0.f;
+    }
+    float M[NR][NC];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const fbpx_residual& R = a.res[i];
+      const int k = p - R.offset;
+      const bool on = i < a.n_res && k >= 0 && k < R.n;
+#pragma unroll
+      for (int f = 0; f < NC; ++f) {
+        M[i][f] = 0.f;
+        if (on && f < R.n_cols && R.jet[f] < 0) M[i][f] = R.col[f][R.col_idx[f] ? R.col_idx[f][k] : k];
+      }
+    }
+    // the POU reduction (pou_point without its affine loads)
     float u[J], g[J];
-    pou_point<S>(wl, pairbuf, wsum, A, B, p, u);
+    pou_point<S>(wl, pairbuf, wsum, nullptr, nullptr, p, u);
+    if (A != nullptr) {
+      float q[J];
+#pragma unroll
+      for (int c = 0; c < J; ++c) q[c] = u[c];
+      leibniz<S>(Aj, Bj, q, u);
+    }
 #pragma unroll
     for (int c = 0; c < J; ++c) {
       ujet[(size_t)c * wl.n_points + p] = u[c];
@@ -860,11 +891,7 @@ __global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const f
       if (k < 0 || k >= R.n) continue;
       float F[NC];
 #pragma unroll
-      for (int f = 0; f < NC; ++f) {
-        F[f] = 0.f;
-        if (f < R.n_cols)
-          F[f] = R.jet[f] >= 0 ? sel<J>(u, R.jet[f]) : R.col[f][R.col_idx[f] ? R.col_idx[f][k] : k];
-      }
+      for (int f = 0; f < NC; ++f) F[f] = (f < R.n_cols && R.jet[f] >= 0) ? sel<J>(u, R.jet[f]) : M[i][f];
       float xa[NT], xb[NT], cf[NT];
       float r = 0.f;
 #pragma unroll
@@ -900,7 +927,7 @@ __global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const f
         }
       }
     }
-    pou_point_T<S>(wl, wsum, B, g, p, gseg);
+    pou_point_T<S>(wl, wsum, B ? Bj : nullptr, g, p, gseg, true);
   }
   // per-residual workgroup sums of r^2
   float tot[NR];

@@ -461,6 +461,15 @@ def _with_grad_trainable(all_params):
     return {"static": all_params["static"], "trainable": rec(all_params["trainable"])}
 
 
+# One launch for the POU reduction, residuals and transpose (fbpx_pou_loss)
+# pays off where the points fit one workgroup; on larger grids it is slower
+# than the separate launches, with the loads that do not depend on the
+# reduction issued first too (steps/s: Burgers' first slab 5,635-5,684
+# against 5,878-5,897, the batched inverse problem 11,867-11,999 against
+# 12,269-12,274, Poisson 2,410-2,419 against 2,431-2,442).
+POU_LOSS_MAX_POINTS = 256
+
+
 class _JetsResidualLoss(torch.autograd.Function):
     """Loss of the fused residual description from the subdomain rows, as one
     autograd node (Epilogue.jets_loss): fbpx_forward -> fbpx_pou_reduce with
@@ -520,13 +529,7 @@ class _JetsResidualLoss(torch.autograd.Function):
         total = torch.empty((), dtype=torch.float32, device=theta.device)
         # unit-seed gradients in the loss launch (see the class docstring)
         pregrad = ep._grad_wanted and ctx.needs_input_grad[0] and not any(ctx.needs_input_grad[4:4 + nr])
-        # one launch for the POU reduction, residuals and transpose pays off
-        # where its points fit one workgroup: a per-point chain of dependent
-        # loads that the separate launches overlap, and on larger grids a
-        # device-scope fence per workgroup (Burgers' first slab: 22.5 against
-        # 5.1 + 8.6 + 4.8 us; the oscillator's 200 points: one launch
-        # instead of three)
-        fused_pou = pregrad and ep.pou_loss_fits and wl.n_points <= 256
+        fused_pou = pregrad and ep.pou_loss_fits and wl.n_points <= POU_LOSS_MAX_POINTS
         gp = []
         if fused_pou:
             rs = _JetsResidualLoss._pou_loss(ep, plan, C, coefs, pvecs, total, gp)
