@@ -365,6 +365,18 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.SIGNATURES) == declared
 
 
+def test_ctypes_signatures_match_header_parameter_counts():
+    """Every ctypes argtypes list has as many entries as the header's
+    declaration has parameters (a binding one argument short would pass
+    garbage for the rest)."""
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "fbpx.h")).read(), flags=re.S)
+    decls = re.findall(r"\b(?:int|int32_t|int64_t|const char\*)\s+(fbpx_\w+)\s*\(([^;]*?)\)\s*;", hdr, flags=re.S)
+    assert len(decls) == len(_lib.SIGNATURES)
+    for name, args in decls:
+        n = 0 if args.strip() in ("", "void") else len(args.split(","))
+        assert len(_lib.SIGNATURES[name][1]) == n, name
+
+
 def test_library_shape_queries_and_error_paths():
     L = _lib.lib()
     assert L.fbpx_version().decode().startswith("fbpx")
