@@ -232,7 +232,10 @@ def pou_loss(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf:
     factor indices), cols (per column: an int jet column, or a tensor of
     per-point data / a gathered vector), idx (per column: None or the int32
     gather index), gcols (per column: None or a per-point gradient tensor),
-    r (the residual tensor).  The tensors stay referenced by the caller."""
+    r (the residual tensor).  The tensors stay referenced by the caller.
+    A plain function rather than a torch custom op: its per-residual
+    descriptors hold lists of tensors, and it is called only inside the
+    fused loss's autograd node."""
     if not 1 <= len(residuals) <= _lib.FBPX_POU_LOSS_RES:
         raise FbpxError(f"pou_loss: {len(residuals)} residuals (at most {_lib.FBPX_POU_LOSS_RES})")
     if (A is None) != (B is None) or (A is not None and (A.shape != ujet.shape or B.shape != ujet.shape)):

@@ -336,7 +336,8 @@ int fbpx_affine_jets_backward(void* stream, const fbpx_net* net, int32_t n, int3
  * gathered column, to the vector it gathers from (a per-instance trainable
  * coefficient indexed by each point's instance); col_idx: NULL or a HOST
  * array of n_cols device int32 index pointers (NULL entries: plain columns,
- * the index is the point's); fa, fb:
+ * the index is the point's; the device does not check the index values,
+ * each of which must address an element of its gathered vector); fa, fb:
  * HOST arrays of n_terms <= FBPX_MAX_TERMS column indices; coef: DEVICE array of n_terms floats (trainable coefficients stay
  * on the device).  Writes r[n] and loss[0]; scratch: FBPX_RESIDUAL_BLOCKS floats.
  * done: NULL (a second one-workgroup launch sums the per-workgroup partials)
@@ -406,7 +407,7 @@ typedef struct {
   int32_t fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
   int32_t jet[FBPX_POU_LOSS_COLS];         /* jet column, or -1: col[f] */
   const float* col[FBPX_POU_LOSS_COLS];    /* DEVICE per-point data (n) or gathered vector */
-  const int32_t* col_idx[FBPX_POU_LOSS_COLS]; /* DEVICE gather index (n) or NULL */
+  const int32_t* col_idx[FBPX_POU_LOSS_COLS]; /* DEVICE gather index (n, each value inside col[f]) or NULL */
   float* gcol[FBPX_POU_LOSS_COLS];         /* DEVICE per-point gradient (n) or NULL */
   float* r;                                /* DEVICE, n */
 } fbpx_residual;
