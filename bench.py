@@ -138,8 +138,13 @@ def matched_l1(name, steps, point_chunks, over):
         torch.cuda.synchronize()
         t_f += time.time() - t0
         t0 = time.time()
-        for _ in range(mk - done):
+        t_log = t0
+        for j in range(mk - done):
             pl.step()
+            if time.time() - t_log >= 30.0:   # a line per half minute of plain steps
+                torch.cuda.synchronize()
+                t_log = time.time()
+                log(f"plain step {done + j + 1} of {steps}")
         torch.cuda.synchronize()
         t_p += time.time() - t0
         done = mk
