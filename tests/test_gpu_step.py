@@ -458,6 +458,19 @@ def _pou_loss_engine(name):
     if name == "batched_inverse":
         import test_gpu_inverse
         return test_gpu_inverse._engine(4)[1]
+    if name == "ho_multilevel":
+        # three levels of subdomains: several POU levels per point (npou > 1)
+        dec_cls, ap = setups.build((5,), (0.0,), (1.0,), 2.3, [1, 32, 1], seed=0, multilevel=True)
+        ap["static"]["domain"] = RectangularDomainND.init_params(np.array([0.0]), np.array([1.0]))[0]
+        prob = P.HarmonicOscillator1DHardBC
+        ap["static"]["problem"] = prob.init_params(d=2, w0=20, sd=0.1)[0]
+        cg = prob.sample_constraints(ap, RectangularDomainND, rng.PRNGKey(7), "grid", ((120,),))
+        eng = FusedEngine(prob, dec_cls, FCN, ap, optimiser_kwargs=dict(learning_rate=1e-3), device="cuda:0",
+                          graph=False)
+        eng.set_constraints([list(c[:-1]) for c in cg], [c[-1] for c in cg])
+        eng.set_active(np.ones(eng.m, dtype=int))
+        assert eng.plan.wl.npou > 1
+        return eng
     if name == "poisson_full":
         from fbpinns_amd import configs
         from fbpinns_amd.trainers import FBPINNTrainer
@@ -467,8 +480,8 @@ def _pou_loss_engine(name):
     return _build(name)[0]
 
 
-@pytest.mark.parametrize("name", ["ho_hard_cfg1", "ho_soft_two_constraints", "burgers_small", "wave_small",
-                                  "poisson_small", "batched_inverse", "poisson_full"])
+@pytest.mark.parametrize("name", ["ho_hard_cfg1", "ho_soft_two_constraints", "ho_multilevel", "burgers_small",
+                                  "wave_small", "poisson_small", "batched_inverse", "poisson_full"])
 def test_pou_loss_launch_equals_separate_kernels(name):
     """fbpx_pou_loss (POU reduction, affine constraint, every residual and the
     POU transpose at unit upstream gradient in one launch) against
