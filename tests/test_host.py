@@ -392,6 +392,53 @@ def test_library_shape_queries_and_error_paths():
     assert rc == -2 and "not compiled" in L.fbpx_last_error().decode()
 
 
+def test_pou_loss_and_residual_grad_validate_arguments():
+    """fbpx_pou_loss and fbpx_residual_loss_grad reject bad arguments with an
+    error code and message before any launch (no GPU needed).  The device
+    pointers are never dereferenced: every case fails validation first."""
+    L = _lib.lib()
+    net = _lib.fbpx_net(1, 32, 1, 1, 1)
+    fake = ctypes.c_void_p(0x1000)
+    wl = _lib.fbpx_worklist()
+    wl.n_points, wl.npou, wl.n_segs = 100, 1, 100
+
+    def pou_loss(res, n_res, scratch_len, A=None, B=None):
+        arr = (_lib.fbpx_residual * max(len(res), 1))(*res)
+        return L.fbpx_pou_loss(None, ctypes.byref(net), ctypes.byref(wl), fake, fake, A, B, n_res, arr, fake, fake,
+                               fake, scratch_len, fake, 0, None)
+
+    def residual(**kw):
+        r = _lib.fbpx_residual()
+        r.offset, r.n, r.n_cols, r.n_terms, r.weight = 0, 100, 1, 1, 1.0
+        r.coef, r.r = fake, fake
+        r.jet[0] = 0
+        for k, v in kw.items():
+            setattr(r, k, v)
+        return r
+    ok = residual()
+    cases = [
+        (([ok], 0, 1), "n_res=0"),
+        (([ok], _lib.FBPX_POU_LOSS_RES + 1, 1), "n_res"),
+        (([ok], 1, 0), "scratch of 0 floats"),
+        (([residual(n=101)], 1, 1), "residual 0: points"),
+        (([residual(n_cols=_lib.FBPX_POU_LOSS_COLS + 1)], 1, 1), "columns"),
+        (([residual(n_terms=0)], 1, 1), "terms"),
+        (([residual(jet=(ctypes.c_int32 * 8)(3))], 1, 1), "jet column 3 of 3"),
+        (([residual(jet=(ctypes.c_int32 * 8)(-1))], 1, 1), "memory column"),
+        (([residual(fa=(ctypes.c_int32 * 8)(1))], 1, 1), "factors (1, 0) outside 1 columns"),
+    ]
+    for (res, n_res, sl), msg in cases:
+        assert pou_loss(res, n_res, sl) == -1, msg
+        assert msg in L.fbpx_last_error().decode(), (msg, L.fbpx_last_error().decode())
+    assert pou_loss([ok], 1, 1, A=fake) == -1 and "both" in L.fbpx_last_error().decode()
+    # the gradient form needs its gradient table
+    fa = (ctypes.c_int32 * 1)(0)
+    cols = (ctypes.c_void_p * 1)(0x1000)
+    assert L.fbpx_residual_loss_grad(None, 10, 1, cols, None, 1, fa, fa, fake, 1.0, fake, fake, fake, 0, None,
+                                     None) == -1
+    assert "gcol" in L.fbpx_last_error().decode()
+
+
 def test_affine_jets_validates_requests():
     L = _lib.lib()
     n = _lib.fbpx_net(2, 32, 2, 3, 1)   # first derivatives along x0, x1; second along x0 only
