@@ -537,3 +537,26 @@ def test_pou_loss_launch_equals_separate_kernels(name):
     e = abs(total1.item() - total0.item()) / abs(total0.item())
     print(f"  loss {total1.item():.9e} / {total0.item():.9e} rel {e:.1e}", flush=True)
     assert e <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["wave_small", "burgers_small", "ho_hard_cfg1"])
+def test_backward_g_scale_scales_each_row(name):
+    """fbpx_backward with g_scale multiplies each chunk's finished gradient row
+    once: equal, bit for bit, to the rows without it times the same scale
+    (1 leaves them unchanged)."""
+    from fbpinns_amd import ops
+    eng, *_ = _build(name)
+    plan = eng.plan
+    net, wl = plan.net, plan.wl
+    gseg = torch.randn(plan.gseg.shape, generator=torch.Generator().manual_seed(3)).cuda()
+    rows = []
+    for sv in (None, 1.0, -1.75):
+        part = torch.full_like(plan.partial, float("nan"))
+        s = None if sv is None else torch.full((), sv, device="cuda")
+        ops.backward(net.as_list(), eng.theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                     gseg, part, s)
+        rows.append(part[:wl.n_chunks])
+    torch.cuda.synchronize()
+    print(f"[g_scale] {name}: {wl.n_chunks} chunk rows", flush=True)
+    assert torch.equal(rows[1], rows[0])
+    assert torch.equal(rows[2], rows[0] * -1.75)
