@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package (`fbpinns_amd`) may
 import, call or execute anything under `oracle/`.  The only permitted users are
-`tests/`, `__graft_entry__.smoke()` (as the checker) and the `cpu_baseline` leg
+`tests/`, `__graft_entry__.smoke()` (to check one step against it) and the `cpu_baseline` leg
 of `bench.py` (as the timed CPU restatement of the method).
 
 Modules

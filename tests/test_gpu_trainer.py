@@ -5,7 +5,7 @@ L1 checks are against the problems' exact solutions:
   * README HO config (HarmonicOscillator1D, d=2, w0=80, 15 subdomains of
     width 0.15, [1,32,1], 200 points, 20,000 steps, lr 1e-3): the reference
     figure (images/test-loss.png via README.md:18) shows FBPINN test L1 of
-    about 2e-3 at the end; this build reaches 1.57e-3 (its own seeded
+    about 2e-3 at the end; the fused engine reaches 1.57e-3 (its own seeded
     initialisation), bound L1 < 2.5e-3.
   * the hard-BC oscillator configuration at its stated 1,000 steps stays
     unconverged (L1 2.65e-1, L1 / std 0.754): held to 5 %, the kernels being

@@ -94,7 +94,7 @@ def test_device_builder_matches_oracle_and_layout(case):
 
 
 def test_reference_self_test_geometry():
-    """decompositions_base.py:87-128 with this build's seeded draws."""
+    """decompositions_base.py:87-128 with seeded numpy draws in place of jax.random."""
     r = np.random.default_rng(0)
     n, m = 10000, 1000
     x = r.uniform(0, 2, (n, 2)).astype(np.float32)
