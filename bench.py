@@ -225,11 +225,13 @@ def host_threads():
     return n
 
 
-def cpu_baseline(c, eng, budget_s=20.0, min_steps=10):
+def cpu_baseline(c, eng, budget_s=20.0, min_steps=10, max_steps=1000):
     """The oracle (torch on CPU, float32) on a bounded sample of the same
     points -- every k-th of the (Z-ordered) training points, so the sample
     spans the domain -- all subdomains, on every CPU this process may use;
-    times min_steps steps and returns steps/s extrapolated by pair count."""
+    times as many steps as fit the budget (min_steps to max_steps: 1,000 for
+    the oscillator's full 200 points, 10-13 on the wave's sample) and
+    returns steps/s extrapolated by pair count."""
     from oracle import adam as oadam
     from oracle import model as om
     from oracle import problems as opb
@@ -283,7 +285,11 @@ def cpu_baseline(c, eng, budget_s=20.0, min_steps=10):
         if t1 * min_steps * 1.5 > budget_s or n == n_full:
             break
         n = min(n_full, int(n * max(1.2, min(8.0, budget_s / (min_steps * max(t1, 1e-3))))))
-    steps = min_steps
+    # the count from a warm step (the first call above includes set-up)
+    t0 = time.time()
+    layers = one_step(layers)
+    t1 = time.time() - t0
+    steps = max(min_steps, min(max_steps, int(budget_s / max(t1, 1e-4))))
     t0 = time.time()
     for _ in range(steps):
         layers = one_step(layers)
