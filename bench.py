@@ -48,12 +48,13 @@ def measured_traffic(config, kernel, launch):
 
 
 def parse_launch(text):
-    """'bins2,chunks512' -> (("bins", 2), ("chunks", 512)): forward, backward."""
+    """'bins2,chunks512' -> (("bins", 2), ("chunks", 512)): forward, backward
+    (wavesC: C-pair chunks, one wave each, for one-hidden-layer networks)."""
     specs = []
     for part in text.split(","):
-        m = re.fullmatch(r"(bins|chunks)(\d+)", part.strip())
+        m = re.fullmatch(r"(bins|chunks|waves)(\d+)", part.strip())
         if not m:
-            raise SystemExit(f"--launch: {part!r} is not binsK or chunksC")
+            raise SystemExit(f"--launch: {part!r} is not binsK, chunksC or wavesC")
         specs.append((m.group(1), int(m.group(2))))
     if len(specs) != 2:
         raise SystemExit("--launch takes two shapes: forward,backward")
@@ -341,7 +342,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape (counter passes see one launch shape)")
     ap.add_argument("--launch", default=None, metavar="F,B",
-                    help="fix the forward,backward launch shapes (binsK / chunksC, e.g. bins2,bins2) "
+                    help="fix the forward,backward launch shapes (binsK / chunksC / wavesC, e.g. bins2,bins2) "
                          "instead of autotuning them")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (counter calibration) and exit")
@@ -414,8 +415,9 @@ def main():
     dom_kernel = "bwd_kernel" if dom_is_bwd else "fwd_kernel"
     # the recorded counters hold for the configuration's own overlap and the
     # launch shape they were measured with
-    launch_txt = (f"fwd {autotune.name(eng.launch[0])} ({eng.plan.wl.n_fbins} workgroups), "
-                  f"bwd {autotune.name(eng.launch[1])} ({eng.plan.wl.n_bbins} workgroups)")
+    wgf, wgb = eng.plan.wl.workgroups()
+    launch_txt = (f"fwd {autotune.name(eng.launch[0])} ({wgf} workgroups), "
+                  f"bwd {autotune.name(eng.launch[1])} ({wgb} workgroups)")
     traffic = measured_traffic(a.config, dom_kernel, launch_txt) if not over else None
 
     plain = None
@@ -474,8 +476,8 @@ def main():
                                f"{eng.plan.wl.n_pairs} (point, subdomain) pairs, FCN [{eng.net.xd}, "
                                f"{eng.net.h}x{eng.net.nh}, 1], jets J={eng.net.J}, hipGraph-captured step",
                    "collocation_points": n_pts, "subdomains": eng.m, "pairs": eng.plan.wl.n_pairs,
-                   "launch": {"forward": f"{autotune.name(eng.launch[0])}, {eng.plan.wl.n_fbins} workgroups",
-                              "backward": f"{autotune.name(eng.launch[1])}, {eng.plan.wl.n_bbins} workgroups"}},
+                   "launch": {"forward": f"{autotune.name(eng.launch[0])}, {wgf} workgroups",
+                              "backward": f"{autotune.name(eng.launch[1])}, {wgb} workgroups"}},
         "collocation_pts_per_s": sps * n_pts,
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP32_MFMA, "traffic": traffic,

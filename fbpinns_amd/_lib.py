@@ -39,6 +39,7 @@ class fbpx_worklist(ctypes.Structure):
         ("pair_point", _c_p), ("pair_seg", _c_p), ("chunk_model", _c_p), ("chunk_start", _c_p),
         ("point_seg_start", _c_p), ("seg_pair_start", _c_p), ("pair_slot", _c_p),
         ("fbin_chunk_start", _c_p), ("bbin_chunk_start", _c_p),
+        ("fwd_wave_bins", _c_i32), ("bwd_wave_bins", _c_i32),
     ]
 
 

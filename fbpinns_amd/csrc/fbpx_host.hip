@@ -60,6 +60,14 @@ static int check_pair_args(const fbpx_net* net, const float* theta, const float*
     set_error("work list has chunks but no bins (n_fbins=%d n_bbins=%d)", wl->n_fbins, wl->n_bbins);
     return FBPX_EINVAL;
   }
+  if ((wl->fwd_wave_bins != 0 && wl->fwd_wave_bins != 1) || (wl->bwd_wave_bins != 0 && wl->bwd_wave_bins != 1)) {
+    set_error("fwd_wave_bins / bwd_wave_bins must be 0 or 1 (got %d, %d)", wl->fwd_wave_bins, wl->bwd_wave_bins);
+    return FBPX_EINVAL;
+  }
+  if ((wl->fwd_wave_bins || wl->bwd_wave_bins) && net->nh != 1) {
+    set_error("one wave per bin runs one-hidden-layer networks only (nh=%d)", net->nh);
+    return FBPX_EINVAL;
+  }
   return FBPX_OK;
 }
 

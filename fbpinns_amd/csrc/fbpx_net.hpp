@@ -44,11 +44,58 @@ struct Small {
   static constexpr int N = BL + 1;
 };
 
-template <int XD, int H, int NH>
+// The threads that run one chunk: the workgroup, or with WAVE one wave (a
+// workgroup's four waves then run four chunks of their own).  A wave's LDS
+// operations are performed in order, so a wavefront-scope fence orders its
+// writes before its reads where the workgroup needs a barrier.
+template <bool WAVE>
+struct Team {
+  __device__ static int tid() { return WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x; }
+  __device__ static int size() { return WAVE ? 64 : (int)blockDim.x; }
+  __device__ static int wave() { return WAVE ? 0 : (int)(threadIdx.x >> 6); }   // within the team
+  __device__ static int nw() { return WAVE ? 1 : (int)(blockDim.x >> 6); }
+  __device__ static int index() { return WAVE ? (int)(threadIdx.x >> 6) : 0; }  // team of the workgroup
+  __device__ static void sync() {
+    if constexpr (WAVE)
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    else
+      __syncthreads();
+  }
+};
+
+// stage_small in two halves, so that a chunk's other first loads can be
+// issued between them: load() issues every global load of the team's share
+// (registers), store() writes them to LDS.  Teams of 64 or more threads (IT
+// covers a 64-thread team; wider teams leave later slots unused).  The
+// one-hidden-layer chunks use it; the two-hidden-layer ones keep the plain
+// loop, whose register schedule their main loops are tuned against.
+template <int XD, int H, int NH, bool WAVE = false>
+struct SmallStage {
+  using L = Layout<XD, H, NH>;
+  using SM = Small<XD, H, NH>;
+  static constexpr int IT = (SM::N + 63) / 64;
+  float r[IT];
+  __device__ void load(const float* th) {
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int i = Team<WAVE>::tid() + j * Team<WAVE>::size();
+      if (i < SM::N) r[j] = th[i < SM::B2 ? i : (i < SM::WL ? L::B2 + (i - SM::B2) : L::WL + (i - SM::WL))];
+    }
+  }
+  __device__ void store(float* ss) const {
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+      const int i = Team<WAVE>::tid() + j * Team<WAVE>::size();
+      if (i < SM::N) ss[i] = r[j];
+    }
+  }
+};
+
+template <int XD, int H, int NH, bool WAVE = false>
 __device__ inline void stage_small(const float* th, float* ss) {
   using L = Layout<XD, H, NH>;
   using SM = Small<XD, H, NH>;
-  for (int i = threadIdx.x; i < SM::N; i += blockDim.x)
+  for (int i = Team<WAVE>::tid(); i < SM::N; i += Team<WAVE>::size())
     ss[i] = th[i < SM::B2 ? i : (i < SM::WL ? L::B2 + (i - SM::B2) : L::WL + (i - SM::WL))];
 }
 
@@ -253,9 +300,9 @@ __device__ inline float tanh_d3(float h) {
 // zt_i[u] = W1[u][i] / sd_i,
 //   zt[i*H + u] = zt_i[u],  zt[XD*H + i*H + u] = -2 zt_i[u],
 //   zt[2*XD*H + i*H + u] = -2 zt_i[u]^2,  zt[3*XD*H + i*H + u] = zt_i[u]^2.
-template <int XD, int H, class SM>
+template <int XD, int H, class SM, bool WAVE = false>
 __device__ inline void stage_zt(const float* ss, const float* isd, float* zt) {
-  for (int t = threadIdx.x; t < XD * H; t += blockDim.x) {
+  for (int t = Team<WAVE>::tid(); t < XD * H; t += Team<WAVE>::size()) {
     const int i = t / H, u = t % H;
     float s = isd[0];
 #pragma unroll
@@ -418,25 +465,48 @@ __device__ inline void pair_s(const LaneLds& v, const float (*h1)[4], const floa
 // ------------------------------------------------------------------ forward
 
 // One chunk of the forward (a run of pairs of one subdomain).
-template <int XD, int H, int NH, int D1M, int D2M>
+template <int XD, int H, int NH, int D1M, int D2M, bool WAVE = false>
 __device__ inline void fwd_chunk(const float* __restrict__ theta, const float* __restrict__ dom,
                                  const float* __restrict__ x, const fbpx_worklist& wl, float* __restrict__ pairbuf,
                                  int chunk) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, NH>;
   using SM = Small<XD, H, NH>;
-  constexpr int J = S::J, NRB = H / 16;
-  __shared__ float ss[SM::N];
-  __shared__ float szt[4 * XD * H];
+  using T = Team<WAVE>;
+  static_assert(!WAVE || NH == 1, "one wave per chunk: one-hidden-layer networks");
+  constexpr int J = S::J, NRB = H / 16, NT = WAVE ? 4 : 1;
+  __shared__ float ss_[NT][SM::N];
+  __shared__ float szt_[NT][4 * XD * H];
   __shared__ __attribute__((aligned(16))) float swt[NH == 2 ? H * H : 1];
   __shared__ __attribute__((aligned(16))) float swr[NH == 2 && S::N1 > 0 ? H * H : 1];
+  float* const ss = ss_[T::index()];
+  float* const szt = szt_[T::index()];
 
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
-  stage_small<XD, H, NH>(th, ss);
-  if constexpr (NH == 2) stage_w2<H>(th + L::W2, nullptr, swt, S::N1 > 0 ? swr : nullptr);
+  const int lane = threadIdx.x & 63, wave = T::wave(), nw = T::nw();
+  const int pl = lane & 15, g = lane >> 4;
+  // the point of each sub-tile is fetched one sub-tile ahead (pair index at
+  // the top of the previous sub-tile, coordinates in its middle), so the two
+  // dependent global loads overlap the previous sub-tile's compute; with one
+  // hidden layer the first sub-tile's pair index is loaded with the
+  // parameter staging and its coordinates before the staging's LDS writes
+  // and barriers
+  float xn[XD];
+  const int k0 = beg + wave * 16 + pl;
+  if constexpr (NH == 1) {
+    const int p0 = wl.pair_point[k0 < end ? k0 : beg];
+    SmallStage<XD, H, NH, WAVE> st;
+    st.load(th);
+#pragma unroll
+    for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)p0 * XD + i];
+    st.store(ss);
+  } else {
+    stage_small<XD, H, NH, WAVE>(th, ss);
+    stage_w2<H>(th + L::W2, nullptr, swt, S::N1 > 0 ? swr : nullptr);
+  }
   Dom d;
   load_dom(dom, model, d);
   float mu[XD], isd[XD];
@@ -445,19 +515,12 @@ __device__ inline void fwd_chunk(const float* __restrict__ theta, const float* _
     mu[i] = (d.xmax[i] + d.xmin[i]) / 2.f;
     isd[i] = 1.f / ((d.xmax[i] - d.xmin[i]) / 2.f);
   }
-  __syncthreads();
-  stage_zt<XD, H, SM>(ss, isd, szt);
-  __syncthreads();
+  T::sync();
+  stage_zt<XD, H, SM, WAVE>(ss, isd, szt);
+  T::sync();
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int pl = lane & 15, g = lane >> 4;
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, swt, swr, pl, g);
-  // the point of each sub-tile is fetched one sub-tile ahead (pair index at
-  // the top of the previous sub-tile, coordinates in its middle), so the two
-  // dependent global loads overlap the previous sub-tile's compute
-  float xn[XD];
-  {
-    const int k0 = beg + wave * 16 + pl;
+  if constexpr (NH == 2) {
     const int p0 = wl.pair_point[k0 < end ? k0 : beg];
 #pragma unroll
     for (int i = 0; i < XD; ++i) xn[i] = x[(size_t)p0 * XD + i];
@@ -588,14 +651,18 @@ __device__ inline void fwd_chunk(const float* __restrict__ theta, const float* _
 
 // Persistent forward launch: workgroup b runs the chunks of forward bin b
 // in order.
-template <int XD, int H, int NH, int D1M, int D2M>
+// WAVE: wave w of workgroup b runs forward bin 4b + w (one-hidden-layer
+// networks; a wave past the last bin has nothing to do).
+template <int XD, int H, int NH, int D1M, int D2M, bool WAVE = false>
 __global__ void __launch_bounds__(256, NH == 2 ? 3 : 2)
 fwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, float* __restrict__ pairbuf) {
-  const int c0 = wl.fbin_chunk_start[blockIdx.x], c1 = wl.fbin_chunk_start[blockIdx.x + 1];
+  const int bin = WAVE ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
+  if (WAVE && bin >= wl.n_fbins) return;
+  const int c0 = wl.fbin_chunk_start[bin], c1 = wl.fbin_chunk_start[bin + 1];
   for (int chunk = c0; chunk < c1; ++chunk) {
-    if (chunk > c0) __syncthreads();   // every wave is done with the previous chunk's LDS
-    fwd_chunk<XD, H, NH, D1M, D2M>(theta, dom, x, wl, pairbuf, chunk);
+    if (chunk > c0) Team<WAVE>::sync();   // the team is done with the previous chunk's LDS
+    fwd_chunk<XD, H, NH, D1M, D2M, WAVE>(theta, dom, x, wl, pairbuf, chunk);
   }
 }
 
@@ -642,13 +709,14 @@ __device__ inline void pair_upstream(const float* gU, const float* xv, const Dom
 // Chunk epilogue shared by both network depths: sums the per-wave small
 // gradients (ssmall[wave][kind][u]; kinds WL, B1, W1[:, i], B2) and the
 // per-lane output-bias partials (sbl) into sacc, then writes the row.
-template <int XD, int H, int NH>
+template <int XD, int H, int NH, bool WAVE = false>
 __device__ inline void finish_row(float* sacc, const float* ssmall, const float* sbl, int nw, float* out,
                                   float gs) {
   using L = Layout<XD, H, NH>;
+  using T = Team<WAVE>;
   constexpr int NS = (NH == 2 ? 3 : 2) + XD;
   constexpr int K_WL = 0, K_B1 = 1, K_W1 = 2;
-  for (int t = threadIdx.x; t < NS * H; t += blockDim.x) {
+  for (int t = T::tid(); t < NS * H; t += T::size()) {
     const int kind = t / H, u = t % H;
     float s = 0.f;
     for (int w = 0; w < nw; ++w) s += ssmall[(w * NS + kind) * H + u];
@@ -659,40 +727,59 @@ __device__ inline void finish_row(float* sacc, const float* ssmall, const float*
     else dst = L::B2 + u;
     sacc[dst] += s;
   }
-  if (threadIdx.x == 0) {
+  if (T::tid() == 0) {
     float s = 0.f;
     for (int q = 0; q < nw * 16; ++q) s += sbl[q];
     sacc[L::BL] = s;
   }
-  __syncthreads();
+  T::sync();
   // gs: d L / d loss for a gseg at unit upstream gradient (the row is linear
   // in gseg; 1 leaves it unchanged)
-  for (int i = threadIdx.x; i < L::P; i += blockDim.x) out[i] = sacc[i] * gs;
+  for (int i = T::tid(); i < L::P; i += T::size()) out[i] = sacc[i] * gs;
 }
 
 // [xd, h, 1]: no hidden-to-hidden layer; waves work independently.
-template <int XD, int H, int D1M, int D2M>
+// WAVE: the chunk is one wave's (its own LDS copies, its slices of the
+// per-wave sums).
+template <int XD, int H, int D1M, int D2M, bool WAVE = false>
 __device__ inline void bwd_one(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
                                const float* gseg, float* partial, int chunk, float gs) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 1>;
   using SM = Small<XD, H, 1>;
-  constexpr int J = S::J, NRB = H / 16, NS = 2 + XD;
+  using T = Team<WAVE>;
+  constexpr int J = S::J, NRB = H / 16, NS = 2 + XD, NT = WAVE ? 4 : 1;
   constexpr int K_WL = 0, K_B1 = 1, K_W1 = 2;
-  __shared__ float ss[SM::N];
-  __shared__ float sacc[L::P];
-  __shared__ float ssmall[4 * NS * H];
-  __shared__ float sbl[64];
-  __shared__ float szt[4 * XD * H];
+  __shared__ float ss_[NT][SM::N];
+  __shared__ float sacc_[NT][L::P];
+  __shared__ float ssmall_[4 * NS * H];   // per wave (four teams of one wave with WAVE)
+  __shared__ float sbl_[64];
+  __shared__ float szt_[NT][4 * XD * H];
+  float* const ss = ss_[T::index()];
+  float* const sacc = sacc_[T::index()];
+  float* const szt = szt_[T::index()];
+  float* const ssmall = ssmall_ + T::index() * NS * H;
+  float* const sbl = sbl_ + T::index() * 16;
 
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
   const int end = wl.chunk_start[chunk + 1];
   const float* th = theta + (size_t)model * L::P;
-  stage_small<XD, H, 1>(th, ss);
-  for (int i = threadIdx.x; i < L::P; i += blockDim.x) sacc[i] = 0.f;
-  for (int i = threadIdx.x; i < 4 * NS * H; i += blockDim.x) ssmall[i] = 0.f;
-  if (threadIdx.x < 64) sbl[threadIdx.x] = 0.f;
+  const int lane = threadIdx.x & 63, wave = T::wave(), nw = T::nw();
+  const int pl = lane & 15, g = lane >> 4;
+  // pair point and segment of this lane's pair in the next sub-tile: loaded
+  // one sub-tile ahead, the first with the parameter staging
+  int pn, sn;
+  {
+    const int k0 = beg + wave * 16 + pl;
+    const int kk = k0 < end ? k0 : beg;
+    pn = wl.pair_point[kk];
+    sn = wl.pair_seg[kk];
+  }
+  stage_small<XD, H, 1, WAVE>(th, ss);
+  for (int i = T::tid(); i < L::P; i += T::size()) sacc[i] = 0.f;
+  for (int i = T::tid(); i < T::nw() * NS * H; i += T::size()) ssmall[i] = 0.f;
+  if (T::tid() < T::nw() * 16) sbl[T::tid()] = 0.f;
   Dom d;
   load_dom(dom, model, d);
   float mu[XD], isd[XD];
@@ -701,21 +788,23 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
     mu[i] = (d.xmax[i] + d.xmin[i]) / 2.f;
     isd[i] = 1.f / ((d.xmax[i] - d.xmin[i]) / 2.f);
   }
-  __syncthreads();
-  stage_zt<XD, H, SM>(ss, isd, szt);
-  __syncthreads();
+  T::sync();
+  stage_zt<XD, H, SM, WAVE>(ss, isd, szt);
+  T::sync();
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int pl = lane & 15, g = lane >> 4;
   const LaneLds v = lane_lds<XD, H, SM>(ss, szt, szt, szt, pl, g);
   float* mysmall = ssmall + wave * NS * H;
   float dbl = 0.f;
   for (int base = beg + wave * 16; base < end; base += nw * 16) {
     const int k = base + pl;
-    const bool valid = k < end;
-    const int kk = valid ? k : beg;
-    const int p = wl.pair_point[kk];
-    const int seg = wl.pair_seg[kk];
+    const bool valid = k < end;   // else the pair at beg with a zero upstream gradient
+    const int p = pn, seg = sn;
+    {
+      const int kn = k + nw * 16;
+      const int kk = kn < end ? kn : beg;
+      pn = wl.pair_point[kk];
+      sn = wl.pair_seg[kk];
+    }
     float xv[XD], xh[XD];
 #pragma unroll
     for (int i = 0; i < XD; ++i) {
@@ -778,8 +867,8 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
     reduce_units_to_lds<NRB>(gz10, mysmall + K_B1 * H, pl, g);
   }
   if (g == 0) sbl[wave * 16 + pl] = dbl;
-  __syncthreads();
-  finish_row<XD, H, 1>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
+  T::sync();
+  finish_row<XD, H, 1, WAVE>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
 }
 
 // W2^T G for one sub-tile: GA[ib] (A operand W2^T[i = 16ib+pl][o = 16ob+4g+ks]
@@ -1316,19 +1405,23 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
 // g_scale: NULL, or d L / d loss when gseg holds the segment gradients at
 // unit upstream gradient; it multiplies each chunk's gradient row as the row
 // is written (finish_row).
-template <int XD, int H, int NH, int D1M, int D2M>
+template <int XD, int H, int NH, int D1M, int D2M, bool WAVE = false>
 __global__ void __launch_bounds__(256, 2)
 bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, const uint8_t* __restrict__ chunk_train, const float* __restrict__ gseg,
            const float* __restrict__ g_scale, float* __restrict__ partial) {
+  static_assert(!WAVE || NH == 1, "one wave per chunk: one-hidden-layer networks");
   const float gs = g_scale ? g_scale[0] : 1.f;
-  // persistent: workgroup b runs the chunks of backward bin b in order
-  const int c0 = wl.bbin_chunk_start[blockIdx.x], c1 = wl.bbin_chunk_start[blockIdx.x + 1];
+  // persistent: workgroup b runs the chunks of backward bin b in order (WAVE:
+  // wave w runs bin 4b + w)
+  const int bin = WAVE ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
+  if (WAVE && bin >= wl.n_bbins) return;
+  const int c0 = wl.bbin_chunk_start[bin], c1 = wl.bbin_chunk_start[bin + 1];
   for (int chunk = c0; chunk < c1; ++chunk) {
     if (chunk_train != nullptr && chunk_train[chunk] == 0) continue;
-    __syncthreads();   // every wave is done with the previous chunk's LDS
+    Team<WAVE>::sync();   // the team is done with the previous chunk's LDS
     if constexpr (NH == 1)
-      bwd_one<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs);
+      bwd_one<XD, H, D1M, D2M, WAVE>(theta, dom, x, wl, gseg, partial, chunk, gs);
     else
       bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs);
   }

@@ -25,7 +25,7 @@ from fbpinns_amd import autotune, ops
 from fbpinns_amd._lib import FBPX_DOM_STRIDE, FBPX_MAX_XD
 from fbpinns_amd.epilogue import Epilogue, schema_masks
 from fbpinns_amd.ops import FBPINNJets, FbpxError, NetSpec
-from fbpinns_amd.takes import device_plan, get_x_batch, spatial_order
+from fbpinns_amd.takes import WorkPlan, device_plan, get_x_batch, spatial_order
 from fbpinns_amd.util.logger import logger
 
 # compiled jet schemas per input dimension (fbpx_shapes.hpp)
@@ -258,8 +258,9 @@ class FusedEngine:
     def set_active(self, active, chunk=None, launch=None):
         """New scheduler state: rebuild takes, work list, epilogue and graph
         (trainers.py:509-575 and the recompile at :647-653).  launch:
-        (forward, backward) launch shapes, each ("chunks", C) or ("bins", k)
-        (see autotune); chunk=C is (("chunks", C), ("chunks", C)).  Default:
+        (forward, backward) launch shapes, each ("chunks", C), ("bins", k) or,
+        for one-hidden-layer networks, ("waves", C) (see autotune); chunk=C
+        is (("chunks", C), ("chunks", C)).  Default:
         autotuned, or autotune.default_spec with autotune off."""
         active = np.asarray(active).astype(np.int64)
         training_ips, cfs, cips, dstat = get_x_batch(active, self.all_params, self.x_global, self.c_offsets,
@@ -282,6 +283,13 @@ class FusedEngine:
         slots = []
 
         def build(fspec, bspec):
+            waves = (fspec[0] == "waves", bspec[0] == "waves")
+            if any(waves):   # the chunk work list, run one wave per bin
+                if self.net.nh != 1:
+                    raise FbpxError(f"one wave per bin runs one-hidden-layer networks only, not {self.net}")
+                p = build(*[("chunks", c[1]) if c[0] == "waves" else c for c in (fspec, bspec)])
+                return WorkPlan(p.wl.with_wave_bins(*waves), p.chunk_train, p.rows, p.row_chunk_start,
+                                p.n_active, p.all_ims)
             if fspec == bspec == small:
                 return plan0
             caps = [c[1] for c in (fspec, bspec) if c[0] == "chunks"]
@@ -299,11 +307,14 @@ class FusedEngine:
             launch = (("chunks", int(chunk)), ("chunks", int(chunk)))
         if launch is not None:
             self.launch, plan = (tuple(launch[0]), tuple(launch[1])), build(tuple(launch[0]), tuple(launch[1]))
-        elif self.autotune_enabled:
-            self.launch, plan = autotune.tune_plan(build, self.net, self.theta, self.dom, x, self.m, n_pairs)
         else:
-            spec = autotune.default_spec(n_pairs)
-            self.launch, plan = (spec, spec), build(spec, spec)
+            n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count
+            if self.autotune_enabled:
+                self.launch, plan = autotune.tune_plan(build, self.net, self.theta, self.dom, x, self.m, n_pairs,
+                                                       n_chunks=plan0.wl.n_chunks, n_cu=n_cu)
+            else:
+                spec = autotune.default_spec(n_pairs, self.net.nh, plan0.wl.n_chunks, n_cu)
+                self.launch, plan = (spec, spec), build(spec, spec)
         wsum = torch.empty((max(plan.wl.n_segs, 1), self.net.J), dtype=torch.float32, device=self.device)
         scratch = torch.empty((max(plan.wl.n_pairs, 1), self.net.J), dtype=torch.float32, device=self.device)
         ops.pou_prepare(self.net.as_list(), self.dom, x, plan.wl.tensor_list(), plan.wl.size_list(), scratch, wsum)
@@ -319,8 +330,8 @@ class FusedEngine:
                                  fused_loss=self.fused_loss)
         self.n_active = n_active
         logger.info(f"work list: {plan.wl.n_points} points, {plan.wl.n_pairs} pairs, {plan.wl.n_chunks} chunks, "
-                    f"fwd {autotune.name(self.launch[0])} ({plan.wl.n_fbins} workgroups), "
-                    f"bwd {autotune.name(self.launch[1])} ({plan.wl.n_bbins} workgroups), "
+                    f"fwd {autotune.name(self.launch[0])} ({plan.wl.workgroups()[0]} workgroups), "
+                    f"bwd {autotune.name(self.launch[1])} ({plan.wl.workgroups()[1]} workgroups), "
                     f"{n_active} trained / {len(all_ims)} used subdomains, jets J={self.net.J}, "
                     f"epilogue={'affine' if self.epilogue.affine is not None else 'generic'}"
                     f"{' + fused residual loss' if self.epilogue.fused is not None else ''}, d={dstat:.2f}")

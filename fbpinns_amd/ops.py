@@ -82,14 +82,16 @@ def _net_from_list(net: List[int]):
 
 _WL_TENSORS = ["pair_point", "pair_seg", "chunk_model", "chunk_start", "point_seg_start", "seg_pair_start",
                "pair_slot", "fbin_chunk_start", "bbin_chunk_start"]
-_WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou", "n_fbins", "n_bbins"]
+_WL_SIZES = ["n_pairs", "n_chunks", "n_points", "n_segs", "npou", "n_fbins", "n_bbins", "fwd_wave_bins",
+             "bwd_wave_bins"]
 
 
 class Worklist:
     """Device-resident work list (see fbpx_worklist in include/fbpx.h).
 
     tensors: dict of int32 CUDA tensors named as in _WL_TENSORS.
-    sizes:   dict of ints named as in _WL_SIZES.
+    sizes:   dict of ints named as in _WL_SIZES (the two wave flags default
+             to 0: one workgroup per bin).
     """
 
     def __init__(self, tensors, sizes):
@@ -104,7 +106,19 @@ class Worklist:
         return [self.t[k] for k in _WL_TENSORS]
 
     def size_list(self):
-        return [self.n[k] for k in _WL_SIZES]
+        return [self.n.get(k, 0) for k in _WL_SIZES]
+
+    def workgroups(self):
+        """(forward, backward) workgroups launched: one per bin, or one per
+        four bins with one wave per bin."""
+        f, b = self.n["n_fbins"], self.n["n_bbins"]
+        return ((f + 3) // 4 if self.n.get("fwd_wave_bins") else f,
+                (b + 3) // 4 if self.n.get("bwd_wave_bins") else b)
+
+    def with_wave_bins(self, fwd, bwd):
+        """The same work list launched with one wave per forward / backward
+        bin (fwd, bwd: bool; one-hidden-layer networks)."""
+        return Worklist(self.t, dict(self.n, fwd_wave_bins=int(bool(fwd)), bwd_wave_bins=int(bool(bwd))))
 
     def c(self):
         return _wl_struct(self.tensor_list(), self.size_list())

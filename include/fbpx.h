@@ -88,6 +88,10 @@ typedef struct {
   const int32_t* fbin_chunk_start; /* [n_fbins+1] chunks of forward workgroup b:
                                       fbin_chunk_start[b] .. fbin_chunk_start[b+1]-1 */
   const int32_t* bbin_chunk_start; /* [n_bbins+1] chunks of backward workgroup b */
+  int32_t fwd_wave_bins;          /* 1: each wave of a forward workgroup runs a bin of its own
+                                     (four bins per workgroup, ceil(n_fbins / 4) workgroups),
+                                     for one-hidden-layer networks; 0: one workgroup per bin */
+  int32_t bwd_wave_bins;          /* the same for the backward */
 } fbpx_worklist;
 
 /* ---------------------------------------------------------------- takes */

@@ -24,11 +24,11 @@ from oracle import takes as ot
 pytestmark = pytest.mark.gpu
 
 
-def _engine(n_inst, graph=False):
+def _engine(n_inst, graph=False, launch=None):
     c = configs.inverse_batch(n_inst=n_inst, graph=graph)
     tr = FBPINNTrainer(c, write_outputs=False)
     eng, sched, x_test, u_exact = tr.setup()
-    eng.set_active(next(iter(sched)))
+    eng.set_active(next(iter(sched)), launch=launch)
     return c, eng, x_test, u_exact
 
 
@@ -146,6 +146,25 @@ def test_instances_are_independent():
     lo = sum(m_per[:2])
     rows = (g1 - g0).abs().sum(1) > 0
     assert rows[lo:lo + m_per[2]].all() and rows.sum().item() == m_per[2]
+
+
+def test_256_instances_wave_bins_match_workgroup_bins():
+    """The 256-instance work list (some 6,000 chunks of about 23 pairs)
+    runs one wave per chunk by default; against one workgroup per 512-pair
+    chunk its loss agrees bit for bit and its gradients (network rows and the
+    256 learnable mu) to float32 summation accuracy."""
+    _, ew, _, _ = _engine(n_inst=256)
+    _, ec, _, _ = _engine(n_inst=256, launch=(("chunks", 512), ("chunks", 512)))
+    print(f"[inverse256 launch] default {ew.launch} bins {ew.plan.wl.n_fbins}, workgroups "
+          f"{ew.plan.wl.workgroups()}; chunks {ec.plan.wl.n_chunks}", flush=True)
+    assert ew.launch == (("waves", 32), ("waves", 32))
+    lw, gw, pw = ew.loss_and_grad()
+    lc, gc, pc = ec.loss_and_grad()
+    e = ((gw - gc).abs().max() / gc.abs().max()).item()
+    ep = ((pw - pc).abs().max() / pc.abs().max()).item()
+    print(f"  loss {lw.item():.8e} / {lc.item():.8e}; grad rel {e:.2e}, mu grad rel {ep:.2e}", flush=True)
+    assert lw.item() == lc.item()
+    assert e < 1e-5 and ep < 1e-5
 
 
 def test_256_instances_train_in_one_launch():

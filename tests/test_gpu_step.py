@@ -16,7 +16,9 @@ import torch
 
 from fbpinns_amd import problems as P
 from fbpinns_amd.domains import RectangularDomainND
+from fbpinns_amd import ops
 from fbpinns_amd.engine import FusedEngine
+from fbpinns_amd.ops import FbpxError
 from fbpinns_amd.networks import FCN
 from fbpinns_amd.util import rng
 
@@ -63,6 +65,12 @@ CASES = {
     "wave_small": (P.WaveEquationConstantVelocity3D, dict(c0=1, source=P.wave_sources()),
                    (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0), (3, 3, 2), 1.9, [3, 64, 64, 1], ((12, 12, 12),),
                    "uniform", lambda: opb.wave(P.wave_sources()), False),
+    # one hidden layer at xd = 2 and 3 (the networks one wave per bin runs)
+    "poisson_one_layer": (P.PoissonMultiscale2D, dict(), (0.0, 0.0), (1.0, 1.0), (4, 4), 1.9, [2, 64, 1],
+                          ((48, 48),), "uniform", lambda: opb.poisson(0.1), False),
+    "wave_one_layer": (P.WaveEquationConstantVelocity3D, dict(c0=1, source=P.wave_sources()),
+                       (-1.0, -1.0, 0.0), (1.0, 1.0, 1.0), (3, 3, 2), 1.9, [3, 16, 1], ((12, 12, 12),),
+                       "uniform", lambda: opb.wave(P.wave_sources()), False),
 }
 
 
@@ -267,6 +275,55 @@ def test_launch_shape_changes_only_summation_order(monkeypatch):
             assert e < 1e-5
             if launch[1][0] == "chunks" and launch[0][0] == "chunks":
                 assert wl.n_chunks >= wl.n_pairs // launch[0][1]
+
+
+@pytest.mark.parametrize("name", ["ho_hard_cfg1", "ho_inverse_mu", "poisson_one_layer", "wave_one_layer"])
+def test_wave_bins_match_workgroup_bins_and_oracle(name):
+    """One wave per bin (one-hidden-layer networks) gives the loss of one
+    workgroup per bin bit for bit (a pair's jets do not depend on which wave
+    computes it) and gradients equal to float32 summation accuracy, for chunk
+    caps from 1 pair up, mixed with the other kernel's workgroup bins, on a
+    persistent-bin work list (several chunks per wave, run in order), and
+    with fixed subdomains (backward chunks skipped); the gradients stay
+    within TOL of the float64 oracle."""
+    eng, ap, constraints, ofac, has_mu = _build(name)
+    shapes = [(("waves", c), ("waves", c)) for c in (1, 64, 256, 2048)]
+    shapes += [(("waves", 256), ("chunks", 256)), (("chunks", 64), ("waves", 64)), (("bins", 1), ("waves", 256))]
+    for active in (np.ones(eng.m, dtype=int), np.where(np.arange(eng.m) % 3 == 0, 2, 1)):
+        eng.set_active(active, chunk=256)
+        l0, g0, gp0 = eng.loss_and_grad()
+        for launch in shapes + ["bins"]:
+            if launch == "bins":   # persistent bins of several chunks, one wave each
+                eng.set_active(active, launch=(("bins", 1), ("bins", 1)))
+                eng.plan.wl = eng.plan.wl.with_wave_bins(True, True)
+            else:
+                eng.set_active(active, launch=launch)
+            wl = eng.plan.wl
+            assert (wl.fwd_wave_bins, wl.bwd_wave_bins) == ((1, 1) if launch == "bins" else
+                                                           tuple(int(s[0] == "waves") for s in launch))
+            l1, g1, gp1 = eng.loss_and_grad()
+            e = ((g1 - g0).abs().max() / g0.abs().max()).item()
+            print(f"[waves] {name} {launch}: chunks {wl.n_chunks} bins {wl.n_fbins}/{wl.n_bbins} "
+                  f"loss {l1.item():.8e} grad rel {e:.2e}", flush=True)
+            assert l1.item() == l0.item()
+            assert e < 1e-5
+            if has_mu:
+                assert abs(gp1[0].item() - gp0[0].item()) <= 1e-5 * abs(gp0[0].item())
+        if (active == 1).all():
+            oloss, ogl, ogx, all_ims, _, _ = _oracle_loss_grad(eng, constraints, ofac, has_mu)
+            _check_layer_grads(g1, ogl, all_ims)
+
+
+def test_wave_bins_rejected_for_two_hidden_layers():
+    eng, *_ = _build("ho_nonaffine_generic")
+    with pytest.raises(FbpxError, match="one-hidden-layer"):
+        eng.set_active(np.ones(eng.m, dtype=int), launch=(("waves", 256), ("chunks", 256)))
+    wl = eng.plan.wl.with_wave_bins(False, True)
+    net, plan = eng.net, eng.plan
+    partial = torch.empty((max(wl.n_chunks, 1), net.P), dtype=torch.float32, device="cuda")
+    with pytest.raises(FbpxError, match="one-hidden-layer"):
+        ops.backward(net.as_list(), eng.theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(),
+                     plan.chunk_train, plan.gseg, partial)
 
 
 def test_scheduler_masks_fixed_rows():

@@ -268,11 +268,54 @@ def test_autotune_tune_plan_builds_every_candidate_it_compares(monkeypatch):
         times[plan.pair[0]][0], times[plan.pair[1]][1]))
     monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
     A._cache.clear()
-    net = type("N", (), {"as_list": lambda self: [3, 64, 2, 7, 7]})()
+    net = type("N", (), {"as_list": lambda self: [3, 64, 2, 7, 7], "nh": 2})()
     (fs, bs), plan = A.tune_plan(build, net, None, None, None, 864, 7_000_000)
     assert plan.pair == (fs, bs)
     assert (fs, bs) == (("bins", 1), ("bins", 1))
     assert (("bins", 2), ("bins", 1)) in built and (("bins", 2), ("chunks", 512)) in built
+    assert not any("waves" in (f[0], b[0]) for f, b in built)
+    A._cache.clear()
+
+
+def test_autotune_wave_bins_default_and_candidate(monkeypatch):
+    """One wave per chunk: the untimed default for a one-hidden-layer work
+    list below MIN_PAIRS with at least WAVE_MIN_CHUNKS_PER_CU chunks per CU
+    (never for two hidden layers), and a timed candidate of one-hidden-layer
+    networks only, which wins when it is fastest."""
+    from fbpinns_amd import autotune as A
+    w = ("waves", A.WAVE_CHUNK)
+    cut = A.WAVE_MIN_CHUNKS_PER_CU * 256
+    assert A.default_spec(141_658, 1, 6078, 256) == w
+    assert A.default_spec(50_000, 1, cut, 256) == w
+    assert A.default_spec(50_000, 1, cut - 1, 256) == A.DEFAULT_SMALL
+    assert A.default_spec(400, 1, 15, 256) == A.DEFAULT_SMALL
+    assert A.default_spec(141_658, 2, 6078, 256) == A.DEFAULT_SMALL
+    assert A.default_spec(A.MIN_PAIRS, 1, 60_000, 256) == A.DEFAULT_LARGE
+    assert w in A.candidates(1) and w not in A.candidates(2)
+
+    class Plan:
+        def __init__(self, f, b):
+            self.pair = (f, b)
+            self.wl = type("W", (), {"n_chunks": 1000 + 10 * f[1] + b[1]})()
+
+    built = []
+
+    def build(f, b):
+        built.append((f, b))
+        return Plan(f, b)
+    net1 = type("N", (), {"as_list": lambda self: [1, 32, 1, 1, 1], "nh": 1})()
+    # below MIN_PAIRS: default_spec with the caller's chunk and CU counts
+    assert A.tune_plan(build, net1, None, None, None, 6000, 141_658, n_chunks=6078, n_cu=256)[0] == (w, w)
+    assert A.tune_plan(build, net1, None, None, None, 15, 400, n_chunks=15, n_cu=256)[0] == (A.DEFAULT_SMALL,) * 2
+    times = {("chunks", 256): (1.00, 3.00), ("chunks", 512): (0.98, 2.88), ("bins", 1): (1.02, 2.90),
+             ("bins", 2): (0.99, 2.95), w: (0.70, 2.00)}
+    monkeypatch.setattr(A, "_time_launches", lambda net, th, dom, x, plan, reps: (
+        times[plan.pair[0]][0], times[plan.pair[1]][1]))
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    A._cache.clear()
+    built.clear()
+    (fs, bs), plan = A.tune_plan(build, net1, None, None, None, 6000, 1_000_000)
+    assert (fs, bs) == (w, w) and plan.pair == (w, w) and (w, w) in built
     A._cache.clear()
 
 
@@ -390,6 +433,15 @@ def test_library_shape_queries_and_error_paths():
     n = _lib.fbpx_net(2, 48, 2, 3, 3)
     rc = L.fbpx_forward(None, ctypes.byref(n), None, None, None, None, None)
     assert rc == -2 and "not compiled" in L.fbpx_last_error().decode()
+    # one wave per bin: flags 0 / 1, one-hidden-layer networks only
+    wl = _lib.fbpx_worklist()
+    for nh, fw, bw, msg in ((2, 1, 0, "one-hidden-layer"), (2, 0, 1, "one-hidden-layer"),
+                            (1, 2, 0, "must be 0 or 1"), (1, 0, -1, "must be 0 or 1")):
+        wl.fwd_wave_bins, wl.bwd_wave_bins = fw, bw
+        n = _lib.fbpx_net(1, 32, nh, 1, 1)
+        rc = L.fbpx_forward(None, ctypes.byref(n), None, None, None, ctypes.byref(wl), None)
+        assert rc == -1 and msg in L.fbpx_last_error().decode(), (nh, fw, bw)
+    assert ctypes.sizeof(_lib.fbpx_worklist) == 7 * 4 + 4 + 9 * 8 + 2 * 4
 
 
 def test_pou_loss_and_residual_grad_validate_arguments():
