@@ -106,6 +106,8 @@ SIGNATURES = {
     "fbpx_pou_loss": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p, _c_p, _c_p,
                                _c_p, _c_i32, ctypes.POINTER(fbpx_residual), _c_p, _c_p, _c_p, ctypes.c_int64, _c_p,
                                _c_i32, _c_p]),
+    "fbpx_residual_losses": (_c_i32, [_c_p, _c_i32, ctypes.POINTER(fbpx_residual), _c_p, ctypes.c_int64, _c_p,
+                                      _c_i32, _c_p]),
     "fbpx_residual_loss_backward": (_c_i32, [_c_p, ctypes.c_int64, _c_i32, ctypes.POINTER(_c_p),
                                              ctypes.POINTER(_c_p), _c_i32, ctypes.POINTER(_c_i32),
                                              ctypes.POINTER(_c_i32), _c_p, _c_p, ctypes.c_float, _c_p,

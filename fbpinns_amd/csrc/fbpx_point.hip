@@ -647,23 +647,21 @@ __device__ inline void residual_col_grads(const ResidualArgs& a, const float* cf
   }
 }
 
-// r[k] = sum_t coef[t] term_t(k) for NT = n_terms; scratch[block] = sum of
-// r^2 over the block's grid-stride points.  GRAD: also the loss's column
-// gradients at unit upstream gradient, gcol[f][k] = w2n r[k] d r / d F(f, k)
-// (the arithmetic of residual_backward_kernel with g_loss = 1).
+// r[k] = sum_t coef[t] term_t(k) for NT = n_terms over the grid-stride
+// points of workgroup lb of a grid of nbl workgroups; returns the sum of r^2
+// over them in thread 0.  GRAD: also the loss's column gradients at unit
+// upstream gradient, gcol[f][k] = w2n r[k] d r / d F(f, k) (the arithmetic of
+// residual_backward_kernel with g_loss = 1).
 template <int NT, bool GRAD>
-__global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
-                                                       float* __restrict__ r, float* scratch, float scale,
-                                                       float* __restrict__ loss, bool acc, int32_t* done,
-                                                       float w2n) {
-  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+__device__ inline float residual_block(int64_t n, const ResidualArgs& a, const float* __restrict__ coef,
+                                       float* __restrict__ r, float w2n, int64_t lb, int64_t nbl, float* sm) {
   const TermCols<NT> c = term_cols<NT>(a);
   float cf[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) cf[t] = coef[t];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t stride = nbl * blockDim.x;
   float s = 0.f;
-  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
+  for (int64_t k0 = lb * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
     float xa[kResUnroll][NT], xb[kResUnroll][NT], sr[kResUnroll];
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
@@ -689,7 +687,19 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
     }
     if constexpr (GRAD) residual_col_grads<NT>(a, cf, xa, xb, sr, k0, stride, n);
   }
-  const float tot = block_tree_sum(s, sm);
+  return block_tree_sum(s, sm);
+}
+
+// residual_block over the grid; scratch[block] = its sum of r^2, summed into
+// loss[0] (scaled, added to it on entry when acc) by the last workgroup when
+// done is given, else by sum_columns_kernel.
+template <int NT, bool GRAD>
+__global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a, const float* __restrict__ coef,
+                                                       float* __restrict__ r, float* scratch, float scale,
+                                                       float* __restrict__ loss, bool acc, int32_t* done,
+                                                       float w2n) {
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  const float tot = residual_block<NT, GRAD>(n, a, coef, r, w2n, blockIdx.x, gridDim.x, sm);
   if (done && gridDim.x == 1) {
     // the only workgroup: its total is the sum that last_block_sum would
     // form (the other threads add zeros), with no fence, atomic or scratch
@@ -698,6 +708,89 @@ __global__ void __launch_bounds__(256) residual_kernel(int64_t n, ResidualArgs a
   }
   if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
   if (done) last_block_sum(scratch, 1, scale, loss, acc, done, sm);
+}
+
+// Several residual losses in one grid: residual i takes workgroups
+// [b0[i], b0[i + 1]) and runs there exactly as residual_kernel runs it on a
+// grid of its own.
+struct ResidualSums {
+  int n_res;
+  int b0[FBPX_POU_LOSS_RES + 1];
+  float scale[FBPX_POU_LOSS_RES];
+};
+
+struct ResidualSet {
+  ResidualSums sums;
+  int nt[FBPX_POU_LOSS_RES];
+  int64_t n[FBPX_POU_LOSS_RES];
+  float w2n[FBPX_POU_LOSS_RES];
+  const float* coef[FBPX_POU_LOSS_RES];
+  float* r[FBPX_POU_LOSS_RES];
+  ResidualArgs a[FBPX_POU_LOSS_RES];
+};
+
+// loss[0] = (loss[0] when acc) + scale[0] S_0 + scale[1] S_1 + ..., S_i the
+// sum of residual i's workgroup partials in sum_columns_kernel's order, each
+// added as a launch of residual_kernel adds it: the loss of the residual
+// launches one after another, bit for bit.  Called by every thread of one
+// workgroup.
+__device__ inline void residual_set_sum(const float* scratch, const ResidualSums& q, float* loss, bool acc,
+                                        float* sm) {
+  float total = acc ? loss[0] : 0.f;
+  for (int i = 0; i < q.n_res; ++i) {
+    float v = 0.f;
+    for (int b = q.b0[i] + threadIdx.x; b < q.b0[i + 1]; b += blockDim.x) v += scratch[b];
+    const float t = block_tree_sum(v, sm);
+    if (threadIdx.x == 0) total = (i == 0 && !acc) ? q.scale[i] * t : total + q.scale[i] * t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[0] = total;
+}
+
+// NTMAX: the largest term count of the set (2, 4 or 8), so that a set of
+// short residuals is not compiled with the registers of eight-term ones.
+template <int NTMAX, bool GRAD>
+__global__ void __launch_bounds__(256) residual_set_kernel(ResidualSet s_arg, float* scratch,
+                                                           float* __restrict__ loss, bool acc, int32_t* done) {
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  __shared__ int last;
+  // the set is read where the launch placed it (the first kernel argument,
+  // at the start of the argument segment): indexed by the residual, a copy
+  // of the argument would be made in per-lane stack memory
+  const ResidualSet& s = *(const ResidualSet*)__builtin_amdgcn_kernarg_segment_ptr();
+  int i = 0;
+  while (i + 1 < s.sums.n_res && (int)blockIdx.x >= s.sums.b0[i + 1]) ++i;
+  const int64_t lb = (int64_t)blockIdx.x - s.sums.b0[i], nbl = s.sums.b0[i + 1] - s.sums.b0[i];
+  float tot = 0.f;
+#define FBPX_SET_CASE(NT)                                                                       \
+  case NT:                                                                                      \
+    if constexpr (NT <= NTMAX)                                                                  \
+      tot = residual_block<NT, GRAD>(s.n[i], s.a[i], s.coef[i], s.r[i], s.w2n[i], lb, nbl, sm); \
+    break;
+  switch (s.nt[i]) {
+    FBPX_SET_CASE(1) FBPX_SET_CASE(2) FBPX_SET_CASE(3) FBPX_SET_CASE(4)
+    FBPX_SET_CASE(5) FBPX_SET_CASE(6) FBPX_SET_CASE(7) FBPX_SET_CASE(8)
+  }
+#undef FBPX_SET_CASE
+  if (threadIdx.x == 0) scratch[blockIdx.x] = tot;
+  if (!done) return;
+  // the last workgroup to finish sums the partials (as last_block_sum)
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();   // this workgroup's partial is visible before it is counted
+    last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();     // every other workgroup's partial is visible here
+  residual_set_sum(scratch, s.sums, loss, acc, sm);
+  if (threadIdx.x == 0) atomicExch(done, 0);
+}
+
+__global__ void __launch_bounds__(256) residual_set_sum_kernel(const float* __restrict__ scratch, ResidualSums q,
+                                                               float* __restrict__ loss, bool acc) {
+  __shared__ float sm[4];
+  residual_set_sum(scratch, q, loss, acc, sm);
 }
 
 // gcol[f][k] = s r[k] d r / d col_f, s = g_loss[0] * w2n; scratch[block *
@@ -1360,6 +1453,69 @@ int fbpx_residual_loss_grad(void* stream, int64_t n, int32_t n_cols, const float
   }
   return residual_loss_launch(stream, n, n_cols, col, col_idx, n_terms, fa, fb, coef, weight, r, scratch, loss,
                               accumulate, done, gcol);
+}
+
+int fbpx_residual_losses(void* stream, int32_t n_res, const fbpx_residual* res, float* scratch,
+                         int64_t scratch_len, float* loss, int32_t accumulate, int32_t* done) {
+  if (n_res < 1 || n_res > FBPX_POU_LOSS_RES || !res || !scratch || !loss) {
+    set_error("residual_losses: n_res=%d (1 .. %d) or a null pointer", n_res, FBPX_POU_LOSS_RES);
+    return FBPX_EINVAL;
+  }
+  ResidualSet s{};
+  s.sums.n_res = n_res;
+  bool grad = false;
+  for (int i = 0; i < n_res; ++i) {
+    const fbpx_residual& R = res[i];
+    if (R.offset != 0 || R.n < 1 || R.n_cols < 1 || R.n_cols > FBPX_POU_LOSS_COLS || !R.coef || !R.r) {
+      set_error("residual_losses: residual %d: offset %d (must be 0), %d points, %d columns (1 .. %d) or a null "
+                "coef / r", i, R.offset, R.n, R.n_cols, FBPX_POU_LOSS_COLS);
+      return FBPX_EINVAL;
+    }
+    for (int f = 0; f < R.n_cols; ++f) {
+      if (R.jet[f] != -1) {
+        set_error("residual_losses: residual %d column %d: jet %d (columns here are device memory, jet -1)", i, f,
+                  R.jet[f]);
+        return FBPX_EINVAL;
+      }
+      grad = grad || R.gcol[f] != nullptr;
+    }
+    int rc = residual_args(R.n, R.n_cols, R.col, R.col_idx, R.n_terms, R.fa, R.fb, R.gcol, &s.a[i]);
+    if (rc) return rc;
+    s.nt[i] = R.n_terms;
+    s.n[i] = R.n;
+    s.coef[i] = R.coef;
+    s.r[i] = R.r;
+    s.sums.scale[i] = R.weight / (float)R.n;
+    s.w2n[i] = 2.f * R.weight / (float)R.n;   // as in fbpx_residual_loss
+    s.sums.b0[i + 1] = s.sums.b0[i] + residual_blocks(R.n);
+  }
+  const int nb = s.sums.b0[n_res];
+  if (scratch_len < nb) {
+    set_error("residual_losses: scratch of %lld floats for %d workgroups", (long long)scratch_len, nb);
+    return FBPX_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const bool acc = accumulate != 0;
+  int32_t* const one = nb <= kSingleLaunchBlocks ? done : nullptr;
+  int ntmax = 0;
+  for (int i = 0; i < n_res; ++i) ntmax = s.nt[i] > ntmax ? s.nt[i] : ntmax;
+#define FBPX_SET_LAUNCH(NTMAX)                                                                                \
+  if (grad)                                                                                                   \
+    hipLaunchKernelGGL((residual_set_kernel<NTMAX, true>), dim3(nb), dim3(256), 0, st, s, scratch, loss, acc, \
+                       one);                                                                                  \
+  else                                                                                                        \
+    hipLaunchKernelGGL((residual_set_kernel<NTMAX, false>), dim3(nb), dim3(256), 0, st, s, scratch, loss, acc, \
+                       one);
+  if (ntmax <= 2) {
+    FBPX_SET_LAUNCH(2)
+  } else if (ntmax <= 4) {
+    FBPX_SET_LAUNCH(4)
+  } else {
+    FBPX_SET_LAUNCH(8)
+  }
+#undef FBPX_SET_LAUNCH
+  if (!one) hipLaunchKernelGGL(residual_set_sum_kernel, dim3(1), dim3(256), 0, st, scratch, s.sums, loss, acc);
+  return hip_check(hipGetLastError(), "residual_losses");
 }
 
 int fbpx_residual_loss_backward(void* stream, int64_t n, int32_t n_cols, const float* const* col,

@@ -147,6 +147,9 @@ class Epilogue:
         self.pou_loss_fits = self.combined is not None and len(self.fused) <= _lib.FBPX_POU_LOSS_RES and all(
             len(keys) <= _lib.FBPX_POU_LOSS_COLS for _, _, keys, *_ in self.fused)
         self._pou_loss_done = (torch.zeros(1, dtype=torch.int32, device=x.device) if self.pou_loss_fits else None)
+        # fbpx_residual_losses runs two or more of them in one launch (the
+        # same limits; one residual keeps its own launch)
+        self.residual_set_fits = self.pou_loss_fits and len(self.fused) >= 2
 
     # -------------------------------------------------------------- set-up
 
@@ -533,6 +536,9 @@ class _JetsResidualLoss(torch.autograd.Function):
         gp = []
         if fused_pou:
             rs = _JetsResidualLoss._pou_loss(ep, plan, C, coefs, pvecs, total, gp)
+        elif ep.residual_set_fits:
+            ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
+            rs = _JetsResidualLoss._residual_set(ep, C, coefs, pvecs, total, gp if pregrad else None)
         else:
             ops.pou_reduce(net.as_list(), wl.tensor_list(), wl.size_list(), plan.pairbuf, plan.wsum, C, A, B)
             rs = []
@@ -557,6 +563,30 @@ class _JetsResidualLoss(torch.autograd.Function):
         ctx.gp = gp
         ep._gC_owner = ctx.token
         return total
+
+    @staticmethod
+    def _residual_set(ep, C, coefs, pvecs, total, gp):
+        """Every residual in one launch (fbpx_residual_losses), each as
+        residual_loss_grad (gp given: the gradient columns at unit upstream
+        gradient, gathered columns' buffers appended to gp) or residual_loss
+        computes it.  Returns the residuals."""
+        it = iter(pvecs)
+        residuals, rs = [], []
+        for i, (ic, weight, keys, data, fa, fb, static, dyn, done) in enumerate(ep.fused):
+            cols, gathered, idx = _JetsResidualLoss._columns(ep, C, i, it)
+            n_c = ep.cranges[ic][1]
+            ix, gc = [None] * len(cols), [None] * len(cols)
+            for j, t in zip(gathered, idx):
+                ix[j] = t
+            if gp is not None:
+                for j, t in zip(*_JetsResidualLoss._gradient_columns(ep, i, gp)):
+                    gc[j] = t
+            rs.append(torch.empty(n_c, dtype=torch.float32, device=C.device))
+            residuals.append(dict(n=n_c, weight=weight, coef=coefs[i], fa=fa, fb=fb, cols=cols, idx=ix, gcols=gc,
+                                  r=rs[-1]))
+        scratch = torch.empty(len(residuals) * _lib.FBPX_RESIDUAL_BLOCKS, dtype=torch.float32, device=C.device)
+        ops.residual_losses(residuals, scratch, total, ep.fused[0][-1])
+        return rs
 
     @staticmethod
     def _pou_loss(ep, plan, C, coefs, pvecs, total, gp):

@@ -236,6 +236,68 @@ def backward(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Te
           "fbpx_backward")
 
 
+def _residual_array(residuals: list, name: str):
+    """ctypes fbpx_residual[len(residuals)] for residual dicts: offset
+    (default 0), n, weight, coef (tensor), fa, fb (term factor indices), cols
+    (per column: an int jet column, or a tensor of per-point data / a gathered
+    vector), idx (per column: None or the int32 gather index), gcols (per
+    column: None or a per-point gradient tensor), r (the residual tensor)."""
+    if not 1 <= len(residuals) <= _lib.FBPX_POU_LOSS_RES:
+        raise FbpxError(f"{name}: {len(residuals)} residuals (at most {_lib.FBPX_POU_LOSS_RES})")
+    arr = (_lib.fbpx_residual * len(residuals))()
+    for d, R in zip(arr, residuals):
+        cols, fa, fb = R["cols"], R["fa"], R["fb"]
+        if not 1 <= len(cols) <= _lib.FBPX_POU_LOSS_COLS or not 1 <= len(fa) <= _lib.FBPX_MAX_TERMS \
+                or len(fb) != len(fa):
+            raise FbpxError(f"{name}: {len(cols)} columns (at most {_lib.FBPX_POU_LOSS_COLS}), {len(fa)}/{len(fb)} "
+                            f"term factors (at most {_lib.FBPX_MAX_TERMS})")
+        n = int(R["n"])
+        if R["r"].numel() != n or R["coef"].numel() != len(fa):
+            raise FbpxError(f"{name}: r of {R['r'].numel()} for {n} points, coef {R['coef'].numel()} for "
+                            f"{len(fa)} terms")
+        d.offset, d.n, d.n_cols, d.n_terms = int(R.get("offset", 0)), n, len(cols), len(fa)
+        d.weight = float(R["weight"])
+        d.coef = _p(R["coef"], torch.float32, "coef")
+        d.r = _p(R["r"], torch.float32, "r")
+        for t, (a, b) in enumerate(zip(fa, fb)):
+            d.fa[t], d.fb[t] = int(a), int(b)
+        for f, c in enumerate(cols):
+            ix, gc = R["idx"][f], R["gcols"][f]
+            if isinstance(c, int):
+                d.jet[f] = c
+            else:
+                d.jet[f] = -1
+                d.col[f] = _p(c, torch.float32, "column")
+                if ix is None and c.numel() != n:
+                    raise FbpxError(f"{name}: column of {c.numel()} values for {n} points")
+            if ix is not None:
+                if ix.numel() != n:
+                    raise FbpxError(f"{name}: gather index of {ix.numel()} entries for {n} points")
+                d.col_idx[f] = _p(ix, torch.int32, "gather index")
+            if gc is not None:
+                if gc.numel() != n:
+                    raise FbpxError(f"{name}: gradient column of {gc.numel()} values for {n} points")
+                d.gcol[f] = _p(gc, torch.float32, "gradient column")
+    return arr
+
+
+def residual_losses(residuals: list, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
+                    accumulate: bool = False) -> None:
+    """fbpx_residual_losses: several residual losses in one launch, each as
+    residual_loss_grad (some gradient column given) or residual_loss would
+    compute it, the loss summed as those launches one after another would.
+    residuals: dicts as for pou_loss, every column a tensor.  scratch: at
+    least len(residuals) * FBPX_RESIDUAL_BLOCKS floats.  A plain function
+    for the same reason as pou_loss."""
+    arr = _residual_array(residuals, "residual_losses")
+    if loss.numel() != 1:
+        raise FbpxError(f"residual_losses: loss of {loss.numel()} values")
+    check(lib().fbpx_residual_losses(_stream(), len(residuals), arr, _p(scratch, torch.float32, "scratch"),
+                                     scratch.numel(), _p(loss, torch.float32, "loss"), int(bool(accumulate)),
+                                     _p(done, torch.int32, "done")),
+          "fbpx_residual_losses")
+
+
 def pou_loss(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf: torch.Tensor, wsum: torch.Tensor,
              A: Optional[torch.Tensor], B: Optional[torch.Tensor], residuals: list, ujet: torch.Tensor,
              gseg: torch.Tensor, scratch: torch.Tensor, loss: torch.Tensor, done: torch.Tensor,
@@ -250,43 +312,9 @@ def pou_loss(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbuf:
     A plain function rather than a torch custom op: its per-residual
     descriptors hold lists of tensors, and it is called only inside the
     fused loss's autograd node."""
-    if not 1 <= len(residuals) <= _lib.FBPX_POU_LOSS_RES:
-        raise FbpxError(f"pou_loss: {len(residuals)} residuals (at most {_lib.FBPX_POU_LOSS_RES})")
     if (A is None) != (B is None) or (A is not None and (A.shape != ujet.shape or B.shape != ujet.shape)):
         raise FbpxError("pou_loss: A and B are both given (shaped as ujet) or both None")
-    arr = (_lib.fbpx_residual * len(residuals))()
-    for d, R in zip(arr, residuals):
-        cols, fa, fb = R["cols"], R["fa"], R["fb"]
-        if not 1 <= len(cols) <= _lib.FBPX_POU_LOSS_COLS or not 1 <= len(fa) <= _lib.FBPX_MAX_TERMS \
-                or len(fb) != len(fa):
-            raise FbpxError(f"pou_loss: {len(cols)} columns (at most {_lib.FBPX_POU_LOSS_COLS}), {len(fa)}/{len(fb)} "
-                            f"term factors (at most {_lib.FBPX_MAX_TERMS})")
-        n = int(R["n"])
-        if R["r"].numel() != n or R["coef"].numel() != len(fa):
-            raise FbpxError(f"pou_loss: r of {R['r'].numel()} for {n} points, coef {R['coef'].numel()} for "
-                            f"{len(fa)} terms")
-        d.offset, d.n, d.n_cols, d.n_terms, d.weight = int(R["offset"]), n, len(cols), len(fa), float(R["weight"])
-        d.coef = _p(R["coef"], torch.float32, "coef")
-        d.r = _p(R["r"], torch.float32, "r")
-        for t, (a, b) in enumerate(zip(fa, fb)):
-            d.fa[t], d.fb[t] = int(a), int(b)
-        for f, c in enumerate(cols):
-            ix, gc = R["idx"][f], R["gcols"][f]
-            if isinstance(c, int):
-                d.jet[f] = c
-            else:
-                d.jet[f] = -1
-                d.col[f] = _p(c, torch.float32, "column")
-                if ix is None and c.numel() != n:
-                    raise FbpxError(f"pou_loss: column of {c.numel()} values for {n} points")
-            if ix is not None:
-                if ix.numel() != n:
-                    raise FbpxError(f"pou_loss: gather index of {ix.numel()} entries for {n} points")
-                d.col_idx[f] = _p(ix, torch.int32, "gather index")
-            if gc is not None:
-                if gc.numel() != n:
-                    raise FbpxError(f"pou_loss: gradient column of {gc.numel()} values for {n} points")
-                d.gcol[f] = _p(gc, torch.float32, "gradient column")
+    arr = _residual_array(residuals, "pou_loss")
     n_ = _net_from_list(net)
     w = _wl_struct(wl_t, wl_n)
     check(lib().fbpx_pou_loss(_stream(), ctypes.byref(n_), ctypes.byref(w), _p(pairbuf, torch.float32, "pairbuf"),

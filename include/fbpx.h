@@ -420,6 +420,24 @@ int fbpx_pou_loss(void* stream, const fbpx_net* net, const fbpx_worklist* wl, co
                   float* ujet, float* gseg, float* scratch, int64_t scratch_len, float* loss, int32_t accumulate,
                   int32_t* done);
 
+/*
+ * Several residual losses in one launch: a problem's loss_fn summing
+ * mean-squared terms over its constraints, e.g. the inverse oscillator's
+ * physics and data terms (fbpinns/problems.py:258-271).  Residual i runs on
+ * workgroups of its own, as fbpx_residual_loss_grad (some gcol[f] given) or
+ * fbpx_residual_loss (every gcol[f] NULL, in every residual) runs it, and r
+ * and gcol equal those launches' bit for bit; so does the loss, which is
+ * formed as those launches would form it one after another (the first adds
+ * to loss[0] on entry when accumulate != 0, each later one adds its term).
+ * res[i]: fbpx_residual with offset 0 and every jet[f] = -1 (all columns
+ * are device memory, col[f] or col[f][col_idx[f][k]]).  scratch: one float
+ * per workgroup, at most n_res * FBPX_RESIDUAL_BLOCKS; done: as in
+ * fbpx_residual_loss (the last workgroup sums the partials when the grid has
+ * at most 256 workgroups, else a second one-workgroup launch does).
+ */
+int fbpx_residual_losses(void* stream, int32_t n_res, const fbpx_residual* res, float* scratch,
+                         int64_t scratch_len, float* loss, int32_t accumulate, int32_t* done);
+
 /* Its transpose.  With s = g_loss[0] * 2 * weight / n (g_loss: DEVICE
  * pointer to d L / d loss):
  *   gcol[f][k] = s * r[k] * d r[k] / d F(f, k)     for every f with gcol[f] != NULL

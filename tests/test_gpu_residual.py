@@ -16,7 +16,9 @@ transpose, ops.ResidualLoss) against the problems' own loss_fn.
 * fbpx_residual_loss_grad (the loss launch that also writes the column
   gradients at unit upstream gradient) against fbpx_residual_loss followed
   by fbpx_residual_loss_backward with g_loss = 1, bit for bit, and
-  fbpx_segment_sum's scale against scaling afterwards.
+  fbpx_segment_sum's scale against scaling afterwards;
+* fbpx_residual_losses (several residuals in one launch) against one
+  launch per residual, bit for bit.
 """
 
 import pytest
@@ -346,3 +348,74 @@ def test_loss_grad_launch_equals_loss_then_transpose(n, n_vec, use_done):
     ops.segment_sum(seg, perm, gs, ref)
     torch.cuda.synchronize()
     assert ((scaled - ref).abs().max() / ref.abs().max()).item() < 1e-6
+
+
+# (points, terms) per residual.  Workgroups of the one launch: 1 + 1 and
+# 50 + 16 (the last workgroup sums the partials), 1,024 + 293 + 69 (a second
+# launch sums them); term counts reach each compiled maximum (2, 4, 8).
+RESIDUAL_SETS = {
+    "two_small": [(1000, 2), (300, 1)],
+    "inverse_sizes": [(51_200, 3), (16_384, 2)],
+    "three_large": [(1_000_003, 4), (300_000, 1), (70_001, 7)],
+    "eight_terms": [(5000, 8), (257, 5), (1, 3), (4096, 2)],
+}
+_FA = [0, 2, -1, 0, 1, 2, 1, 0]
+_FB = [-1, 1, -1, 0, 1, 0, -1, 2]
+
+
+@pytest.mark.parametrize("acc", [False, True])
+@pytest.mark.parametrize("grad", [True, False])
+@pytest.mark.parametrize("name", list(RESIDUAL_SETS))
+def test_residual_set_launch_equals_separate_launches(name, grad, acc):
+    """fbpx_residual_losses: several residuals (two plain columns and one
+    gathered through a per-point index, up to eight terms) in one launch give
+    r, the column gradients at unit upstream gradient and the loss of
+    fbpx_residual_loss_grad (or fbpx_residual_loss) launched once per
+    residual, each adding into the loss, bit for bit."""
+    from fbpinns_amd import _lib, ops
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(len(name) * 7 + grad + 2 * acc)
+    spec = RESIDUAL_SETS[name]
+    print(f"[residual set] {name} grad={grad} acc={acc}: {spec}", flush=True)
+    data = []
+    for n, nt in spec:
+        n_vec = max(1, n // 200)
+        cols = [torch.randn(n, generator=g).to(dev), torch.randn(n, generator=g).to(dev),
+                torch.randn(n_vec, generator=g).to(dev)]
+        idx = torch.randint(0, n_vec, (n,), generator=g).to(dev, torch.int32)
+        coef = torch.randn(nt, generator=g).to(dev)
+        weight = float(torch.rand((), generator=g)) + 0.5
+        data.append((n, nt, cols, idx, coef, weight))
+    outs = {}
+    for form in ("set", "separate"):
+        loss = torch.full((), 0.25, device=dev)
+        done = torch.zeros(1, dtype=torch.int32, device=dev)
+        rs, gcs, residuals = [], [], []
+        for n, nt, cols, idx, coef, weight in data:
+            rs.append(torch.full((n,), float("nan"), device=dev))
+            gcs.append([torch.full((n,), float("nan"), device=dev), None,
+                        torch.full((n,), float("nan"), device=dev)] if grad else [None] * 3)
+            residuals.append(dict(n=n, weight=weight, coef=coef, fa=_FA[:nt], fb=_FB[:nt], cols=cols,
+                                  idx=[None, None, idx], gcols=gcs[-1], r=rs[-1]))
+        if form == "set":
+            scratch = torch.empty(len(spec) * _lib.FBPX_RESIDUAL_BLOCKS, device=dev)
+            ops.residual_losses(residuals, scratch, loss, done, accumulate=acc)
+        else:
+            for i, R in enumerate(residuals):
+                scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS, device=dev)
+                args = (R["cols"], R["fa"], R["fb"], R["coef"], R["weight"], R["r"], scratch, loss, done, [2],
+                        [R["idx"][2]], acc or i > 0)
+                if grad:
+                    ops.residual_loss_grad(*args, [0, 2], [R["gcols"][0], R["gcols"][2]])
+                else:
+                    ops.residual_loss(*args)
+        torch.cuda.synchronize()
+        assert int(done.item()) == 0
+        outs[form] = (loss, rs, gcs)
+    (l1, r1, g1), (l0, r0, g0) = outs["set"], outs["separate"]
+    print(f"  loss {l1.item():.9e} / {l0.item():.9e}", flush=True)
+    assert torch.equal(l1, l0)
+    for i in range(len(spec)):
+        assert torch.equal(r1[i], r0[i]), i
+        for a, b in zip(g1[i], g0[i]):
+            assert (a is None) == (b is None) and (a is None or torch.equal(a, b)), i

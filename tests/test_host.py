@@ -490,6 +490,25 @@ def test_pou_loss_and_residual_grad_validate_arguments():
                                      None) == -1
     assert "gcol" in L.fbpx_last_error().decode()
 
+    # fbpx_residual_losses: memory columns only, offset 0, a scratch float per workgroup
+    def losses(res, n_res, scratch_len):
+        arr = (_lib.fbpx_residual * max(len(res), 1))(*res)
+        return L.fbpx_residual_losses(None, n_res, arr, fake, scratch_len, fake, 0, None)
+    mem = dict(jet=(ctypes.c_int32 * 8)(-1), col=(ctypes.c_void_p * 8)(0x1000))
+    cases = [
+        (([residual(**mem)], 0, 8), "n_res=0"),
+        (([residual(**mem)] * 5, _lib.FBPX_POU_LOSS_RES + 1, 8), "n_res=5"),
+        (([residual()], 1, 8), "jet 0 (columns here are device memory"),
+        (([residual(offset=3, **mem)], 1, 8), "offset 3 (must be 0)"),
+        (([residual(**mem), residual(n=0, **mem)], 2, 8), "residual 1: offset 0 (must be 0), 0 points"),
+        (([residual(r=None, **mem)], 1, 8), "null coef / r"),
+        (([residual(**mem), residual(**mem)], 2, 1), "scratch of 1 floats for 2 workgroups"),
+        (([residual(fa=(ctypes.c_int32 * 8)(1), **mem)], 1, 8), "factors (1, 0) outside 1 columns"),
+    ]
+    for (res, n_res, sl), msg in cases:
+        assert losses(res, n_res, sl) == -1, msg
+        assert msg in L.fbpx_last_error().decode(), (msg, L.fbpx_last_error().decode())
+
 
 def test_affine_jets_validates_requests():
     L = _lib.lib()
