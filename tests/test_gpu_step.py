@@ -314,6 +314,28 @@ def test_wave_bins_match_workgroup_bins_and_oracle(name):
             _check_layer_grads(g1, ogl, all_ims)
 
 
+def test_autotune_times_wave_bins_for_one_hidden_layer(monkeypatch):
+    """With the tuning threshold at 0 pairs the autotuner times the wave
+    candidate on a one-hidden-layer work list next to the workgroup shapes;
+    whichever it picks gives the loss of 512-pair chunks bit for bit."""
+    from fbpinns_amd import autotune
+    monkeypatch.setattr(autotune, "MIN_PAIRS", 0)
+    autotune._cache.clear()
+    eng, *_ = _build("poisson_one_layer")
+    active = np.ones(eng.m, dtype=int)
+    eng.set_active(active, chunk=512)
+    l0, g0, _ = eng.loss_and_grad()
+    eng.set_active(active)
+    wl = eng.plan.wl
+    print(f"[autotune waves] picked {eng.launch}, bins {wl.n_fbins}/{wl.n_bbins}, workgroups {wl.workgroups()}",
+          flush=True)
+    assert eng.launch[0] in autotune.candidates(1) and eng.launch[1] in autotune.candidates(1)
+    l1, g1, _ = eng.loss_and_grad()
+    assert l1.item() == l0.item()
+    assert ((g1 - g0).abs().max() / g0.abs().max()).item() < 1e-5
+    autotune._cache.clear()
+
+
 def test_wave_bins_rejected_for_two_hidden_layers():
     eng, *_ = _build("ho_nonaffine_generic")
     with pytest.raises(FbpxError, match="one-hidden-layer"):

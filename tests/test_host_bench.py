@@ -44,3 +44,15 @@ def test_traffic_figure_comes_from_the_profile_summary():
     assert len(recs) >= 2
     for k, v in recs:
         assert any(abs(round(fe * 1.99995 * 1024 + wr * f_write * 1024) - int(v)) <= 1 for kk, fe, wr in rows if kk == k), k
+
+
+def test_launch_option_parses_every_shape():
+    import bench
+    assert bench.parse_launch("bins2,chunks512") == (("bins", 2), ("chunks", 512))
+    assert bench.parse_launch("waves32, waves32") == (("waves", 32), ("waves", 32))
+    for bad in ("bins2", "bins2,threads4", "waves,chunks5", "bins1,bins1,bins1"):
+        try:
+            bench.parse_launch(bad)
+        except SystemExit:
+            continue
+        raise AssertionError(bad)
