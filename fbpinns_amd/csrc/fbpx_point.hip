@@ -437,9 +437,10 @@ __device__ inline void adam_count_step(int32_t* count, int32_t* count_done) {
   }
 }
 
-// grad != NULL: the gradient rows; else the per-chunk partial rows, summed
-// per listed row as fbpx_reduce_rows sums them (ascending chunks, same
-// unrolling), so that both forms give the same update bit for bit.
+// PARTIAL false: the gradient rows grad; true: the per-chunk partial rows,
+// summed per listed row as fbpx_reduce_rows sums them (ascending chunks,
+// same unrolling), so that both forms give the same update bit for bit.
+template <bool PARTIAL>
 __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ theta, float* __restrict__ mu,
                                                         float* __restrict__ nu, int P,
                                                         const float* __restrict__ grad,
@@ -460,16 +461,20 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int row = (int)(idx / P), e = (int)(idx % P);
-    const size_t i = (size_t)rows[row] * P + e;
+    // two dependent round trips: the row and its chunk range, then the
+    // parameter, its moments and the gradient (or the chunk rows) together
+    const int r = rows[row];
+    const int c0 = PARTIAL ? rcs[row] : 0, c1 = PARTIAL ? rcs[row + 1] : 0;
+    const size_t i = (size_t)r * P + e;
+    float p = theta[i], m = mu[i], v = nu[i];
     float gi;
-    if (grad != nullptr) {
+    if constexpr (!PARTIAL) {
       gi = grad[i];
     } else {
       gi = 0.f;
 #pragma unroll 4
-      for (int c = rcs[row]; c < rcs[row + 1]; ++c) gi += partial[(size_t)c * P + e];
+      for (int c = c0; c < c1; ++c) gi += partial[(size_t)c * P + e];
     }
-    float p = theta[i], m = mu[i], v = nu[i];
     adam_elem(p, m, v, gi, bc1, bc2, k);
     theta[i] = p;
     mu[i] = m;
@@ -1222,9 +1227,14 @@ static int adam_rows_launch(void* stream, float* theta, float* mu, float* nu, in
   // few workgroups, by a second one-thread launch otherwise
   const int64_t total = (int64_t)n_rows * P, nb = (total + 255) / 256;
   const bool fold = count_done && nb <= kSingleLaunchBlocks;
-  hipLaunchKernelGGL(adam_rows_kernel, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, theta, mu, nu, P,
-                     grad, partial, rcs, rows, n_rows, count, fold ? count_done : nullptr,
-                     adam_const(lr, b1, b2, eps));
+  if (grad)
+    hipLaunchKernelGGL(adam_rows_kernel<false>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, theta, mu,
+                       nu, P, grad, partial, rcs, rows, n_rows, count, fold ? count_done : nullptr,
+                       adam_const(lr, b1, b2, eps));
+  else
+    hipLaunchKernelGGL(adam_rows_kernel<true>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, theta, mu, nu,
+                       P, grad, partial, rcs, rows, n_rows, count, fold ? count_done : nullptr,
+                       adam_const(lr, b1, b2, eps));
   int rc = hip_check(hipGetLastError(), "adam_rows");
   if (rc == FBPX_OK && count_done && !fold) rc = fbpx_count_increment(stream, count);
   return rc;
