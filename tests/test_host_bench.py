@@ -56,3 +56,15 @@ def test_launch_option_parses_every_shape():
         except SystemExit:
             continue
         raise AssertionError(bad)
+
+
+def test_default_steps_give_about_a_second_of_timed_work():
+    """Per-configuration default step counts: the wave keeps the 100 steps
+    the default run times; the others time about a second of their steps."""
+    import bench
+    assert bench.DEFAULT_STEPS["wave"] == (100, 10)
+    step_ms = {"poisson": 0.42, "burgers": 0.17, "ho": 0.025, "inverse": 0.06}
+    for cfg, ms in step_ms.items():
+        k, w = bench.DEFAULT_STEPS[cfg]
+        assert 0.3 <= k * ms / 1e3 <= 2.0, cfg
+        assert 10 <= w <= k // 10, cfg
