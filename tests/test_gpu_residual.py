@@ -352,12 +352,17 @@ def test_loss_grad_launch_equals_loss_then_transpose(n, n_vec, use_done):
 
 # (points, terms) per residual.  Workgroups of the one launch: 1 + 1 and
 # 50 + 16 (the last workgroup sums the partials), 1,024 + 293 + 69 (a second
-# launch sums them); term counts reach each compiled maximum (2, 4, 8).
+# launch sums them), 128 + 128 and 128 + 129 (either side of the 256 where the
+# second launch takes over), one residual alone; term counts reach each
+# compiled maximum (2, 4, 8).
 RESIDUAL_SETS = {
     "two_small": [(1000, 2), (300, 1)],
     "inverse_sizes": [(51_200, 3), (16_384, 2)],
     "three_large": [(1_000_003, 4), (300_000, 1), (70_001, 7)],
     "eight_terms": [(5000, 8), (257, 5), (1, 3), (4096, 2)],
+    "grid_256": [(131_072, 2), (131_072, 3)],
+    "grid_257": [(131_072, 2), (131_073, 3)],
+    "one_residual": [(70_001, 3)],
 }
 _FA = [0, 2, -1, 0, 1, 2, 1, 0]
 _FB = [-1, 1, -1, 0, 1, 0, -1, 2]
