@@ -34,7 +34,8 @@ their step is launch-bound and the timings would be noise.
 
 default_spec for a small work list: ("waves", WAVE_CHUNK) for a
 one-hidden-layer network with at least WAVE_MIN_CHUNKS_PER_CU chunks (at
-FBPX_CHUNK pairs) per CU, else DEFAULT_SMALL.  Measured on MI355X, batched
+FBPX_CHUNK pairs) per CU averaging at most WAVE_MAX_CHUNK_PAIRS pairs (the
+short chunks it was measured on), else DEFAULT_SMALL.  Measured on MI355X, batched
 1D inverse oscillators, [1, 32, 1], 23 pairs per chunk on average, steps/s of
 ("chunks", 512) -> ("waves", 32): 90 chunks 27,100 -> 27,400; 355 chunks
 25,200 -> 25,000; 719 chunks 22,300 -> 22,900; 1,490 chunks 20,700 ->
@@ -60,6 +61,7 @@ DEFAULT_SMALL = ("chunks", FBPX_CHUNK)
 DEFAULT_LARGE = ("bins", 1)
 MIN_PAIRS = 200_000
 WAVE_MIN_CHUNKS_PER_CU = 2
+WAVE_MAX_CHUNK_PAIRS = 64
 TIE = 0.02
 _cache = {}
 
@@ -69,7 +71,7 @@ def default_spec(n_pairs, nh=2, n_chunks=0, n_cu=1):
     FBPX_CHUNK pairs; n_cu: the GPU's CU count."""
     if n_pairs >= MIN_PAIRS:
         return DEFAULT_LARGE
-    if nh == 1 and n_chunks >= WAVE_MIN_CHUNKS_PER_CU * n_cu:
+    if nh == 1 and n_chunks >= WAVE_MIN_CHUNKS_PER_CU * n_cu and n_pairs <= WAVE_MAX_CHUNK_PAIRS * n_chunks:
         return ("waves", WAVE_CHUNK)
     return DEFAULT_SMALL
 

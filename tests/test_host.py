@@ -286,8 +286,12 @@ def test_autotune_wave_bins_default_and_candidate(monkeypatch):
     w = ("waves", A.WAVE_CHUNK)
     cut = A.WAVE_MIN_CHUNKS_PER_CU * 256
     assert A.default_spec(141_658, 1, 6078, 256) == w
-    assert A.default_spec(50_000, 1, cut, 256) == w
-    assert A.default_spec(50_000, 1, cut - 1, 256) == A.DEFAULT_SMALL
+    assert A.default_spec(30_000, 1, cut, 256) == w
+    assert A.default_spec(30_000, 1, cut - 1, 256) == A.DEFAULT_SMALL
+    # long chunks (here 100 pairs on average) keep a workgroup per chunk
+    assert A.default_spec(100 * 6000, 1, 6000, 256) == A.DEFAULT_LARGE
+    assert A.default_spec(100 * 1500, 1, 1500, 256) == A.DEFAULT_SMALL
+    assert A.default_spec(A.WAVE_MAX_CHUNK_PAIRS * 1500, 1, 1500, 256) == w
     assert A.default_spec(400, 1, 15, 256) == A.DEFAULT_SMALL
     assert A.default_spec(141_658, 2, 6078, 256) == A.DEFAULT_SMALL
     assert A.default_spec(A.MIN_PAIRS, 1, 60_000, 256) == A.DEFAULT_LARGE
