@@ -226,10 +226,15 @@ def test_adam_matches_oracle():
     assert int(eng2.count.item()) == 1 and int(eng.count.item()) == 3
 
 
-@pytest.mark.parametrize("name", ["ho_inverse_mu", "wave_small", "burgers_small"])
-def test_graph_replay_equals_eager(name):
+@pytest.mark.parametrize("name,launch", [("ho_inverse_mu", None), ("wave_small", None), ("burgers_small", None),
+                                         ("poisson_one_layer", (("waves", 32), ("waves", 32)))])
+def test_graph_replay_equals_eager(name, launch):
     ea, *_ = _build(name, graph=False)
     eb, *_ = _build(name, graph=True)
+    if launch is not None:   # one wave per chunk (the graph is captured again)
+        for e in (ea, eb):
+            e.set_active(np.ones(e.m, dtype=int), launch=launch)
+        assert eb.plan.wl.fwd_wave_bins == eb.plan.wl.bwd_wave_bins == 1
     assert eb._graph is not None
     junk = torch.randn(1 << 20, device="cuda")
     for k in range(4):
