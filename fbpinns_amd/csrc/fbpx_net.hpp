@@ -796,6 +796,10 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   float* mysmall = ssmall + wave * NS * H;
   float dbl = 0.f;
   for (int base = beg + wave * 16; base < end; base += nw * 16) {
+    // one wave per chunk: the weights and window tangents are read from LDS
+    // per sub-tile rather than held in registers across the loop (95 VGPRs
+    // instead of 120 at [1, 32, 1]: five waves per SIMD instead of four)
+    if constexpr (WAVE) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int k = base + pl;
     const bool valid = k < end;   // else the pair at beg with a zero upstream gradient
     const int p = pn, seg = sn;
