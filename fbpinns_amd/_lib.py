@@ -116,6 +116,8 @@ SIGNATURES = {
                                             _c_i32, ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32), _c_p,
                                             ctypes.c_float, _c_p, _c_p, _c_i32]),
     "fbpx_segment_sum": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "fbpx_pou_backward_segment_sum": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), ctypes.POINTER(fbpx_worklist), _c_p,
+                                               _c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_p, _c_p]),
     "fbpx_param_count": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_supported": (_c_i32, [ctypes.POINTER(fbpx_net)]),
     "fbpx_last_error": (ctypes.c_char_p, []),

@@ -292,19 +292,25 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
 // g_scale != NULL: g_ujet is scaled by g_scale[0] as it is read (d L / d
 // loss applied to jets computed at unit upstream gradient).
 template <class S>
-__global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
-                                                           const float* __restrict__ B,
-                                                           const float* __restrict__ g_ujet,
-                                                           const float* __restrict__ g_scale,
-                                                           float* __restrict__ gseg) {
+__device__ inline void pou_backward_point(const fbpx_worklist& wl, const float* __restrict__ wsum,
+                                          const float* __restrict__ B, const float* __restrict__ g_ujet,
+                                          const float* __restrict__ g_scale, float* __restrict__ gseg, int p) {
   constexpr int J = S::J;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wl.n_points) return;
   const float sc = g_scale ? g_scale[0] : 1.f;
   float gq[J];
 #pragma unroll
   for (int c = 0; c < J; ++c) gq[c] = g_ujet[(size_t)c * wl.n_points + p] * sc;
   pou_point_T<S>(wl, wsum, B, gq, p, gseg);
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) pou_backward_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
+                                                           const float* __restrict__ B,
+                                                           const float* __restrict__ g_ujet,
+                                                           const float* __restrict__ g_scale,
+                                                           float* __restrict__ gseg) {
+  pou_backward_point<S>(wl, wsum, B, g_ujet, g_scale, gseg, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ affine constraining function
@@ -872,17 +878,42 @@ __global__ void __launch_bounds__(256) sum_columns_kernel(const float* __restric
 // out[s] = sum over j in [seg_start[s], seg_start[s + 1]) of x[perm[j]] (or
 // x[j]), times scale[0] when scale != NULL; one block per segment, fixed
 // order.
-__global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restrict__ seg_start,
-                                                          const int32_t* __restrict__ perm,
-                                                          const float* __restrict__ x,
-                                                          const float* __restrict__ scale, float* __restrict__ out) {
-  __shared__ float sm[4];   // one partial per wave of the 256-thread block
-  const int s = blockIdx.x;
+__device__ inline void segment_sum_block(const int32_t* __restrict__ seg_start, const int32_t* __restrict__ perm,
+                                         const float* __restrict__ x, const float* __restrict__ scale,
+                                         float* __restrict__ out, int s, float* sm) {
   const int j0 = seg_start[s], j1 = seg_start[s + 1];
   float v = 0.f;
   for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) v += x[perm ? perm[j] : j];
   const float t = block_tree_sum(v, sm);
   if (threadIdx.x == 0) out[s] = scale ? t * scale[0] : t;
+}
+
+__global__ void __launch_bounds__(256) segment_sum_kernel(const int32_t* __restrict__ seg_start,
+                                                          const int32_t* __restrict__ perm,
+                                                          const float* __restrict__ x,
+                                                          const float* __restrict__ scale, float* __restrict__ out) {
+  __shared__ float sm[4];   // one partial per wave of the 256-thread block
+  segment_sum_block(seg_start, perm, x, scale, out, blockIdx.x, sm);
+}
+
+// pou_backward_kernel on workgroups [0, nb_pou) and segment_sum_kernel on
+// the n_seg after them, each with its own arithmetic: two independent
+// transposes of a fused loss's backward in one launch.
+template <class S>
+__global__ void __launch_bounds__(256) pou_backward_segsum_kernel(fbpx_worklist wl, const float* __restrict__ wsum,
+                                                                  const float* __restrict__ B,
+                                                                  const float* __restrict__ g_ujet,
+                                                                  const float* __restrict__ g_scale,
+                                                                  float* __restrict__ gseg, int nb_pou,
+                                                                  const int32_t* __restrict__ seg_start,
+                                                                  const int32_t* __restrict__ perm,
+                                                                  const float* __restrict__ x,
+                                                                  float* __restrict__ out) {
+  __shared__ float sm[4];
+  if ((int)blockIdx.x < nb_pou)
+    pou_backward_point<S>(wl, wsum, B, g_ujet, g_scale, gseg, blockIdx.x * blockDim.x + threadIdx.x);
+  else
+    segment_sum_block(seg_start, perm, x, g_scale, out, (int)blockIdx.x - nb_pou, sm);
 }
 
 // ------------------------------------------------------------------ POU reduction + residual losses + transpose
@@ -1635,6 +1666,29 @@ int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, cons
   hipLaunchKernelGGL(segment_sum_kernel, dim3(n_seg), dim3(256), 0, (hipStream_t)stream, seg_start, perm, x, scale,
                      out);
   return hip_check(hipGetLastError(), "segment_sum");
+}
+
+int fbpx_pou_backward_segment_sum(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* wsum,
+                                  const float* B, const float* g_ujet, const float* g_scale, float* gseg,
+                                  int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
+                                  float* out) {
+  int J, rc;
+  if ((rc = check_net_schema(net, &J)) || (rc = check_wl(wl))) return rc;
+  if (n_seg < 0 || (n_seg > 0 && (!seg_start || !x || !out))) {
+    set_error("pou_backward_segment_sum: n_seg=%d, null seg_start / x / out", n_seg);
+    return FBPX_EINVAL;
+  }
+  const int nb_pou = (wl->n_points + 255) / 256;
+  if (nb_pou + n_seg == 0) return FBPX_OK;
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((unsigned)(nb_pou + n_seg));
+#define X(XD, D1, D2)                                                                                             \
+  if (net->xd == XD && net->d1mask == D1 && net->d2mask == D2)                                                    \
+    hipLaunchKernelGGL((pou_backward_segsum_kernel<Schema<XD, D1, D2>>), grid, dim3(256), 0, s, *wl, wsum, B,     \
+                       g_ujet, g_scale, gseg, nb_pou, seg_start, perm, x, out);
+  FBPX_SCHEMAS(X)
+#undef X
+  return hip_check(hipGetLastError(), "pou_backward_segment_sum");
 }
 
 int fbpx_mean(void* stream, const float* x, int64_t n, float* scratch, float* out) {

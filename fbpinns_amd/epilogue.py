@@ -667,18 +667,24 @@ class _JetsResidualLoss(torch.autograd.Function):
             ops.residual_loss_backward(cols, list(fa), list(fb), coefs[i], rs[i], weight, g, gidx, gcols, scratch,
                                        gcoef, want, done, gathered, idx)
             gcoefs[i] = gcoef if want else None
-        # transposes of the gathers: per trainable vector, its points summed in index order
-        gvecs = []
-        for d, gcol, nv in zip(pk, gp, ctx.nvec):
-            out = torch.empty(nv, dtype=torch.float32, device=g.device)
-            ops.segment_sum(d[3], d[2], gcol, out, scale)
-            gvecs.append(out)
+        # transposes of the gathers: per trainable vector, its points summed in
+        # index order; a single one shares the POU transpose's launch
+        gvecs = [torch.empty(nv, dtype=torch.float32, device=g.device) for nv in ctx.nvec]
+        with_pou = not (pregrad and ctx.fused_pou) and len(gvecs) == 1
+        for j, (d, gcol, out) in enumerate(zip(pk, gp, gvecs)):
+            if not with_pou:
+                ops.segment_sum(d[3], d[2], gcol, out, scale)
         if pregrad and ctx.fused_pou:
             ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                          plan.gseg, plan.partial, g)
         else:
-            ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg,
-                             ep.affine[1], scale)
+            if with_pou:
+                d = pk[0]
+                ops.pou_backward_segment_sum(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC,
+                                             plan.gseg, ep.affine[1], scale, d[3], d[2], gp[0], gvecs[0])
+            else:
+                ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg,
+                                 ep.affine[1], scale)
             ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
                          plan.gseg, plan.partial)
         return (ops.theta_gradient(plan, theta), None, None, None, *gcoefs, *gvecs)

@@ -611,6 +611,30 @@ def segment_sum(seg_start: torch.Tensor, perm: torch.Tensor, x: torch.Tensor, ou
           "fbpx_segment_sum")
 
 
+@torch.library.custom_op("fbpx::pou_backward_segment_sum", mutates_args=("gseg", "out"))
+def pou_backward_segment_sum(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum: torch.Tensor,
+                             g_ujet: torch.Tensor, gseg: torch.Tensor, B: Optional[torch.Tensor],
+                             g_scale: Optional[torch.Tensor], seg_start: torch.Tensor, perm: torch.Tensor,
+                             x: torch.Tensor, out: torch.Tensor) -> None:
+    """pou_backward(..., B, g_scale) and segment_sum(seg_start, perm, x, out,
+    g_scale) in one launch, bit for bit."""
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    n_seg = out.numel()
+    if B is not None and B.shape != g_ujet.shape:
+        raise FbpxError(f"pou_backward_segment_sum: B {tuple(B.shape)} for jets {tuple(g_ujet.shape)}")
+    if seg_start.numel() != n_seg + 1 or perm.numel() != x.numel() or (g_scale is not None and g_scale.numel() != 1):
+        raise FbpxError(f"pou_backward_segment_sum: {seg_start.numel()} offsets for {n_seg} segments, "
+                        f"{perm.numel()} permutation entries for {x.numel()} values"
+                        + ("" if g_scale is None else f", g_scale of {g_scale.numel()} values"))
+    check(lib().fbpx_pou_backward_segment_sum(
+        _stream(), ctypes.byref(n), ctypes.byref(w), _p(wsum, torch.float32, "wsum"), _p(B, torch.float32, "B"),
+        _p(g_ujet, torch.float32, "g_ujet"), _p(g_scale, torch.float32, "g_scale"), _p(gseg, torch.float32, "gseg"),
+        n_seg, _p(seg_start, torch.int32, "seg_start"), _p(perm, torch.int32, "perm"), _p(x, torch.float32, "x"),
+        _p(out, torch.float32, "out")),
+        "fbpx_pou_backward_segment_sum")
+
+
 class GatherRows(torch.autograd.Function):
     """y[k] = v[idx[k]] for a trainable vector v; the backward sums the
     incoming gradient per index with fbpx_segment_sum over the points sorted

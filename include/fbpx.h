@@ -481,6 +481,16 @@ int fbpx_residual_loss_noalloc(void* stream, int64_t n, int32_t n_cols, const fl
 int fbpx_segment_sum(void* stream, int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
                      const float* scale, float* out);
 
+/* fbpx_pou_backward and fbpx_segment_sum in one launch, each on workgroups
+ * of its own with its own arithmetic (outputs equal the two calls' bit for
+ * bit), both scaled by g_scale: the backward of a fused loss with a
+ * gathered trainable vector, whose two transposes are independent.  No
+ * counterpart in the reference beyond those of the two calls. */
+int fbpx_pou_backward_segment_sum(void* stream, const fbpx_net* net, const fbpx_worklist* wl, const float* wsum,
+                                  const float* B, const float* g_ujet, const float* g_scale, float* gseg,
+                                  int32_t n_seg, const int32_t* seg_start, const int32_t* perm, const float* x,
+                                  float* out);
+
 /* ---------------------------------------------------------------- misc */
 
 /* Parameters per subdomain row for a network shape (0 if unsupported). */

@@ -145,3 +145,21 @@ def test_pou_reduce_with_affine_map_equals_separate_kernels(prob, kw, xd, lo, hi
         gseg_r = torch.empty_like(plan.gseg)
         ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC * sv, gseg_r, B)
         assert torch.equal(gseg_s, gseg_r), sv
+    # the POU transpose and a gathered vector's segment sum in one launch:
+    # each output equals its own call's bit for bit, with and without g_scale
+    g = torch.Generator().manual_seed(2)
+    n_vec = 37
+    idx = torch.randint(0, n_vec, (x.shape[0],), generator=g)
+    perm = torch.argsort(idx, stable=True).to(torch.int32).cuda()
+    seg_start = torch.zeros(n_vec + 1, dtype=torch.int32)
+    seg_start[1:] = torch.cumsum(torch.bincount(idx, minlength=n_vec), 0)
+    seg_start = seg_start.cuda()
+    gv = torch.randn(x.shape[0], generator=g).cuda()
+    for s in (None, torch.full((), -2.75, device="cuda")):
+        gseg_1, out_1 = torch.empty_like(plan.gseg), torch.empty(n_vec, device="cuda")
+        ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC, gseg_1, B, s)
+        ops.segment_sum(seg_start, perm, gv, out_1, s)
+        gseg_2, out_2 = torch.full_like(plan.gseg, float("nan")), torch.full((n_vec,), float("nan"), device="cuda")
+        ops.pou_backward_segment_sum(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, gC, gseg_2, B, s,
+                                     seg_start, perm, gv, out_2)
+        assert torch.equal(gseg_1, gseg_2) and torch.equal(out_1, out_2), s
