@@ -85,6 +85,9 @@ SIGNATURES = {
     "fbpx_occupancy": (_c_i32, [ctypes.POINTER(fbpx_net), ctypes.POINTER(_c_i32), ctypes.POINTER(_c_i32)]),
     "fbpx_backward": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
                                _c_p, _c_p, _c_p, _c_p]),
+    "fbpx_backward_adam": (_c_i32, [_c_p, ctypes.POINTER(fbpx_net), _c_p, _c_p, _c_p, ctypes.POINTER(fbpx_worklist),
+                                    _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double]),
     "fbpx_reduce_rows": (_c_i32, [_c_p, _c_i32, _c_p, _c_p, _c_p, _c_i32, _c_p]),
     "fbpx_adam_rows": (_c_i32, [_c_p, _c_p, _c_p, _c_p, _c_i32, _c_p, _c_p, _c_i32, _c_p, _c_p,
                                 _c_d, _c_d, _c_d, _c_d]),

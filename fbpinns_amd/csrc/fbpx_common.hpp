@@ -189,6 +189,55 @@ __device__ inline void jet_mul_T(const float* g, const float* w, float* gu) {
   gu[0] = g0;
 }
 
+// optax.adam (scale_by_adam, eps_root = 0, then scale_by_learning_rate and
+// apply_updates), element-wise in FP32 with the same operation order:
+//   mu = omb1*g + b1*mu ; nu = omb2*g^2 + b2*nu ; c = count + 1
+//   p  = p + nlr * ((mu / (1 - b1^c)) / (sqrt(nu / (1 - b2^c)) + eps))
+// omb1 = f32(1 - b1) and nlr = f32(-lr) are formed in double on the host, as
+// Python forms them before they meet the FP32 arrays.
+struct AdamConst {
+  float b1, b2, omb1, omb2, nlr, eps;
+};
+
+inline AdamConst adam_const(double lr, double b1, double b2, double eps) {
+  AdamConst k;
+  k.b1 = (float)b1;
+  k.b2 = (float)b2;
+  k.omb1 = (float)(1.0 - b1);
+  k.omb2 = (float)(1.0 - b2);
+  k.nlr = (float)(-lr);
+  k.eps = (float)eps;
+  return k;
+}
+
+// The optimizer step folded into the backward (mu == NULL: off): each
+// chunk's gradient row is applied to its subdomain row of theta in place.
+struct AdamFold {
+  float* theta;
+  float* mu;
+  float* nu;
+  int32_t* count;
+  int32_t* count_done;
+  AdamConst k;
+};
+
+// bc[0] = 1 - b1^c, bc[1] = 1 - b2^c for c = count + 1
+__device__ inline void adam_bias(const int32_t* count, const AdamConst& k, float* bc) {
+  const float cf = (float)(count[0] + 1);
+  bc[0] = 1.f - powf(k.b1, cf);
+  bc[1] = 1.f - powf(k.b2, cf);
+}
+
+__device__ inline void adam_elem(float& p, float& m, float& v, float g, float bc1, float bc2, const AdamConst& k) {
+#pragma clang fp contract(off)
+  m = k.omb1 * g + k.b1 * m;
+  v = k.omb2 * (g * g) + k.b2 * v;
+  const float mh = m / bc1;
+  const float vh = v / bc2;
+  const float u = mh / (sqrtf(vh) + k.eps);
+  p = p + k.nlr * u;
+}
+
 // Thread-local error text; defined in fbpx_host.hip.
 void set_error(const char* fmt, ...);
 int hip_check(hipError_t e, const char* what);

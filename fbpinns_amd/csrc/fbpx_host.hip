@@ -110,6 +110,27 @@ int fbpx_backward(void* stream, const fbpx_net* net, const float* theta, const f
   return dispatch(fns, net, a, (hipStream_t)stream);
 }
 
+int fbpx_backward_adam(void* stream, const fbpx_net* net, float* theta, const float* dom, const float* x,
+                       const fbpx_worklist* wl, const uint8_t* chunk_train, const float* gseg, const float* g_scale,
+                       float* mu, float* nu, int32_t* count, int32_t* count_done, double lr, double b1, double b2,
+                       double eps) {
+  int rc = check_pair_args(net, theta, dom, x, wl, theta);
+  if (rc) return rc;
+  if (net->nh != 1) {
+    set_error("backward_adam: one-hidden-layer networks only (nh=%d)", net->nh);
+    return FBPX_EINVAL;
+  }
+  if (!mu || !nu || !count || !count_done || (wl->n_chunks > 0 && (!gseg || !wl->pair_seg))) {
+    set_error("backward_adam: null mu / nu / count / count_done / gseg / pair_seg");
+    return FBPX_EINVAL;
+  }
+  if (wl->n_chunks == 0) return fbpx_count_increment(stream, count);
+  static const launch_fn fns[3] = {launch_bwd_xd1, launch_bwd_xd2, launch_bwd_xd3};
+  NetArgs a{theta, dom, x, *wl, chunk_train, gseg, nullptr, g_scale,
+            AdamFold{theta, mu, nu, count, count_done, adam_const(lr, b1, b2, eps)}};
+  return dispatch(fns, net, a, (hipStream_t)stream);
+}
+
 int fbpx_occupancy(const fbpx_net* net, int32_t* fwd_per_cu, int32_t* bwd_per_cu) {
   int rc = check_net(net);
   if (rc) return rc;

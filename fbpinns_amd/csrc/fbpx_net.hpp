@@ -709,9 +709,9 @@ __device__ inline void pair_upstream(const float* gU, const float* xv, const Dom
 // Chunk epilogue shared by both network depths: sums the per-wave small
 // gradients (ssmall[wave][kind][u]; kinds WL, B1, W1[:, i], B2) and the
 // per-lane output-bias partials (sbl) into sacc, then writes the row.
-template <int XD, int H, int NH, bool WAVE = false>
+template <int XD, int H, int NH, bool WAVE = false, bool ADAM = false>
 __device__ inline void finish_row(float* sacc, const float* ssmall, const float* sbl, int nw, float* out,
-                                  float gs) {
+                                  float gs, const AdamFold& af, int model, const float* bc) {
   using L = Layout<XD, H, NH>;
   using T = Team<WAVE>;
   constexpr int NS = (NH == 2 ? 3 : 2) + XD;
@@ -735,15 +735,32 @@ __device__ inline void finish_row(float* sacc, const float* ssmall, const float*
   T::sync();
   // gs: d L / d loss for a gseg at unit upstream gradient (the row is linear
   // in gseg; 1 leaves it unchanged)
-  for (int i = T::tid(); i < L::P; i += T::size()) out[i] = sacc[i] * gs;
+  if constexpr (!ADAM) {
+    for (int i = T::tid(); i < L::P; i += T::size()) out[i] = sacc[i] * gs;
+    return;
+  }
+  // the row is its subdomain's whole gradient: Adam on it in place, as
+  // fbpx_adam_rows_partial updates a row from its one chunk row
+  const size_t r0 = (size_t)model * L::P;
+  for (int i = T::tid(); i < L::P; i += T::size()) {
+    const float row = sacc[i] * gs;
+    float gi = 0.f;
+    gi += row;
+    float p = af.theta[r0 + i], m = af.mu[r0 + i], v = af.nu[r0 + i];
+    adam_elem(p, m, v, gi, bc[0], bc[1], af.k);
+    af.theta[r0 + i] = p;
+    af.mu[r0 + i] = m;
+    af.nu[r0 + i] = v;
+  }
 }
 
 // [xd, h, 1]: no hidden-to-hidden layer; waves work independently.
 // WAVE: the chunk is one wave's (its own LDS copies, its slices of the
 // per-wave sums).
-template <int XD, int H, int D1M, int D2M, bool WAVE = false>
+template <int XD, int H, int D1M, int D2M, bool WAVE = false, bool ADAM = false>
 __device__ inline void bwd_one(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
-                               const float* gseg, float* partial, int chunk, float gs) {
+                               const float* gseg, float* partial, int chunk, float gs, const AdamFold& af,
+                               const float* bc) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 1>;
   using SM = Small<XD, H, 1>;
@@ -872,7 +889,7 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   }
   if (g == 0) sbl[wave * 16 + pl] = dbl;
   T::sync();
-  finish_row<XD, H, 1, WAVE>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
+  finish_row<XD, H, 1, WAVE, ADAM>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs, af, model, bc);
 }
 
 // W2^T G for one sub-tile: GA[ib] (A operand W2^T[i = 16ib+pl][o = 16ob+4g+ks]
@@ -1012,7 +1029,8 @@ __device__ inline void store_tile(float* tl, const f4* v) {
 // [xd, h, h, 1]: hidden-to-hidden layer on MFMA, streamed one axis at a time.
 template <int XD, int H, int D1M, int D2M>
 __device__ inline void bwd_two(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
-                               const float* gseg, float* partial, int chunk, float gs) {
+                               const float* gseg, float* partial, int chunk, float gs, const AdamFold& af,
+                               const float* bc) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 2>;
   using SM = Small<XD, H, 2>;
@@ -1403,19 +1421,32 @@ __device__ inline void bwd_two(const float* theta, const float* dom, const float
     }
     __syncthreads();
   }
-  finish_row<XD, H, 2>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs);
+  finish_row<XD, H, 2>(sacc, ssmall, sbl, nw, partial + (size_t)chunk * L::P, gs, af, model, bc);
 }
 
 // g_scale: NULL, or d L / d loss when gseg holds the segment gradients at
 // unit upstream gradient; it multiplies each chunk's gradient row as the row
 // is written (finish_row).
-template <int XD, int H, int NH, int D1M, int D2M, bool WAVE = false>
+// ADAM (one-hidden-layer networks): every trained subdomain row is one
+// chunk, and each chunk's row is applied to theta by Adam (af) in place of
+// being written to partial; the team (workgroup, or wave with WAVE) that
+// finishes last advances the count.
+template <int XD, int H, int NH, int D1M, int D2M, bool WAVE = false, bool ADAM = false>
 __global__ void __launch_bounds__(256, 2)
 bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const float* __restrict__ x,
            fbpx_worklist wl, const uint8_t* __restrict__ chunk_train, const float* __restrict__ gseg,
-           const float* __restrict__ g_scale, float* __restrict__ partial) {
+           const float* __restrict__ g_scale, float* __restrict__ partial, AdamFold af) {
   static_assert(!WAVE || NH == 1, "one wave per chunk: one-hidden-layer networks");
+  static_assert(!ADAM || NH == 1, "the folded optimizer step: one-hidden-layer networks");
   const float gs = g_scale ? g_scale[0] : 1.f;
+  // the bias corrections, from the count as every team reads it before the
+  // last one advances it (the barrier completes the read)
+  __shared__ float sbc[2];
+  if constexpr (ADAM) {
+    if (threadIdx.x == 0) adam_bias(af.count, af.k, sbc);
+    __syncthreads();
+  }
+  const float* const bc = sbc;   // read in LDS where a row is applied
   // persistent: workgroup b runs the chunks of backward bin b in order (WAVE:
   // wave w runs bin 4b + w)
   const int bin = WAVE ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
@@ -1425,9 +1456,15 @@ bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
     if (chunk_train != nullptr && chunk_train[chunk] == 0) continue;
     Team<WAVE>::sync();   // the team is done with the previous chunk's LDS
     if constexpr (NH == 1)
-      bwd_one<XD, H, D1M, D2M, WAVE>(theta, dom, x, wl, gseg, partial, chunk, gs);
+      bwd_one<XD, H, D1M, D2M, WAVE, ADAM>(theta, dom, x, wl, gseg, partial, chunk, gs, af, bc);
     else
-      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs);
+      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs, af, bc);
+  }
+  if constexpr (ADAM) {
+    if (Team<WAVE>::tid() == 0 && atomicAdd(af.count_done, 1) == wl.n_bbins - 1) {
+      atomicAdd(af.count, 1);
+      atomicExch(af.count_done, 0);
+    }
   }
 }
 

@@ -390,26 +390,7 @@ bool schema_supported(int xd, int d1, int d2) {
 
 // ------------------------------------------------------------------ Adam
 //
-// optax.adam (scale_by_adam, eps_root = 0, then scale_by_learning_rate and
-// apply_updates), element-wise in FP32 with the same operation order:
-//   mu = omb1*g + b1*mu ; nu = omb2*g^2 + b2*nu ; c = count + 1
-//   p  = p + nlr * ((mu / (1 - b1^c)) / (sqrt(nu / (1 - b2^c)) + eps))
-// omb1 = f32(1 - b1) and nlr = f32(-lr) are formed in double on the host, as
-// Python forms them before they meet the FP32 arrays.
-
-struct AdamConst {
-  float b1, b2, omb1, omb2, nlr, eps;
-};
-
-__device__ inline void adam_elem(float& p, float& m, float& v, float g, float bc1, float bc2, const AdamConst& k) {
-#pragma clang fp contract(off)
-  m = k.omb1 * g + k.b1 * m;
-  v = k.omb2 * (g * g) + k.b2 * v;
-  const float mh = m / bc1;
-  const float vh = v / bc2;
-  const float u = mh / (sqrtf(vh) + k.eps);
-  p = p + k.nlr * u;
-}
+// (optax.adam's element update, AdamConst and adam_bias: fbpx_common.hpp)
 
 __global__ void __launch_bounds__(256) reduce_rows_kernel(int P, const float* __restrict__ partial,
                                                           const int32_t* __restrict__ rows,
@@ -456,11 +437,7 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
                                                         int32_t* count, int32_t* count_done, AdamConst k) {
   // bias corrections once per workgroup
   __shared__ float sbc[2];
-  if (threadIdx.x == 0) {
-    const float cf = (float)(count[0] + 1);
-    sbc[0] = 1.f - powf(k.b1, cf);
-    sbc[1] = 1.f - powf(k.b2, cf);
-  }
+  if (threadIdx.x == 0) adam_bias(count, k, sbc);
   __syncthreads();
   const float bc1 = sbc[0], bc2 = sbc[1];
   const int64_t total = (int64_t)n_rows * P;
@@ -493,11 +470,7 @@ __global__ void __launch_bounds__(256) adam_flat_kernel(float* __restrict__ p, f
                                                         float* __restrict__ nu, const float* __restrict__ g, int n,
                                                         int32_t* count, int32_t* count_done, AdamConst k) {
   __shared__ float sbc[2];
-  if (threadIdx.x == 0) {
-    const float cf = (float)(count[0] + 1);
-    sbc[0] = 1.f - powf(k.b1, cf);
-    sbc[1] = 1.f - powf(k.b2, cf);
-  }
+  if (threadIdx.x == 0) adam_bias(count, k, sbc);
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -1107,16 +1080,6 @@ __global__ void count_increment_kernel(int32_t* count) {
   if (threadIdx.x == 0) count[0] = count[0] + 1;
 }
 
-static AdamConst adam_const(double lr, double b1, double b2, double eps) {
-  AdamConst k;
-  k.b1 = (float)b1;
-  k.b2 = (float)b2;
-  k.omb1 = (float)(1.0 - b1);
-  k.omb2 = (float)(1.0 - b2);
-  k.nlr = (float)(-lr);
-  k.eps = (float)eps;
-  return k;
-}
 
 }  // namespace fbpx
 

@@ -29,6 +29,7 @@ struct NetArgs {
   const float* gseg;
   float* out;  // pairbuf (forward) or partial (backward)
   const float* g_scale = nullptr;  // backward: NULL or d L / d loss for unit-seed gseg
+  AdamFold adam{};                 // backward: the optimizer step folded in (adam.mu != NULL)
 };
 
 // Return FBPX_EUNSUPPORTED if the shape is not compiled, else launch status.

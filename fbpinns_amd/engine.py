@@ -84,7 +84,7 @@ class FusedEngine:
     has a residual description (its loss is then order-independent)."""
 
     def __init__(self, problem, decomposition, network, all_params, optimiser_kwargs=None, device="cuda:0",
-                 graph=True, autotune=True, fused_loss=True, sort_points=True):
+                 graph=True, autotune=True, fused_loss=True, sort_points=True, fold_adam=True):
         from fbpinns_amd.decompositions import RectangularDecompositionND
         from fbpinns_amd.networks import FCN
         if not torch.cuda.is_available():
@@ -99,6 +99,7 @@ class FusedEngine:
         self.autotune_enabled = bool(autotune)
         self.fused_loss = bool(fused_loss)
         self.sort_points = bool(sort_points)
+        self.fold_adam = bool(fold_adam)
         ok = dict(optimiser_kwargs or {})
         self.lr = float(ok.pop("learning_rate", 1e-3))
         self.b1 = float(ok.pop("b1", 0.9))
@@ -322,6 +323,14 @@ class FusedEngine:
         self.plan = ops.Plan(self.net, self.dom, x, plan.wl, wsum, plan.chunk_train, plan.rows,
                              plan.row_chunk_start, self.m)
         self.plan.keep_gradient_buffer(self.net.P)
+        # the step's Adam inside the backward's launch (fbpx_backward_adam):
+        # one hidden layer, no trainable problem leaves, and every trained
+        # row exactly one chunk (so each chunk's row is its whole gradient)
+        rcs = plan.row_chunk_start.cpu()
+        if (self.fold_adam and self.net.nh == 1 and not self.Q and len(rcs) > 1
+                and bool(((rcs[1:] - rcs[:-1]) == 1).all())):
+            self.plan.adam_fold = (self.theta, self.mu, self.nu, self.count, self.count_done, self.lr, self.b1,
+                                   self.b2, self.eps)
         self.workplan = plan
         self.active = active_
         self.x = x
@@ -360,8 +369,10 @@ class FusedEngine:
     def _step_body(self):
         plan = self.plan
         # the backward leaves the per-chunk gradient rows in plan.partial;
-        # adam_rows_partial sums and applies them in one launch
+        # adam_rows_partial sums and applies them in one launch (or, with
+        # plan.adam_fold, the backward applies Adam itself)
         plan.defer_rows = True
+        plan.adam_folded = False
         th = self.theta.detach().requires_grad_(True)
         params, pleaf = self._params_for_step()
         loss = self.epilogue.jets_loss(th, plan, params)
@@ -370,8 +381,9 @@ class FusedEngine:
         grads = torch.autograd.grad(loss, leaves, grad_outputs=(self.loss_seed,), allow_unused=True)
         g_p = grads[1] if pleaf is not None else None
         # the step's last Adam launch also advances the shared count
-        ops.adam_rows_partial(self.theta, self.mu, self.nu, plan.partial, plan.rows, plan.row_chunk_start,
-                              self.count, self.count_done, not self.Q, self.lr, self.b1, self.b2, self.eps)
+        if not plan.adam_folded:
+            ops.adam_rows_partial(self.theta, self.mu, self.nu, plan.partial, plan.rows, plan.row_chunk_start,
+                                  self.count, self.count_done, not self.Q, self.lr, self.b1, self.b2, self.eps)
         if self.Q:
             if g_p is None:
                 g_p = torch.zeros_like(self.pflat)

@@ -675,8 +675,7 @@ class _JetsResidualLoss(torch.autograd.Function):
             if not with_pou:
                 ops.segment_sum(d[3], d[2], gcol, out, scale)
         if pregrad and ctx.fused_pou:
-            ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
-                         plan.gseg, plan.partial, g)
+            _JetsResidualLoss._backward(plan, theta, g)
         else:
             if with_pou:
                 d = pk[0]
@@ -685,6 +684,19 @@ class _JetsResidualLoss(torch.autograd.Function):
             else:
                 ops.pou_backward(net.as_list(), wl.tensor_list(), wl.size_list(), plan.wsum, ep.gC, plan.gseg,
                                  ep.affine[1], scale)
-            ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
-                         plan.gseg, plan.partial)
+            _JetsResidualLoss._backward(plan, theta, None)
         return (ops.theta_gradient(plan, theta), None, None, None, *gcoefs, *gvecs)
+
+    @staticmethod
+    def _backward(plan, theta, g_scale):
+        """fbpx_backward into plan.partial, or, in the engine's step when it
+        allows it (plan.adam_fold), with the optimizer step folded in."""
+        net, wl = plan.net, plan.wl
+        if plan.defer_rows and plan.adam_fold is not None:
+            th, mu, nu, count, count_done, lr, b1, b2, eps = plan.adam_fold
+            ops.backward_adam(net.as_list(), th, plan.dom, plan.x, wl.tensor_list(), wl.size_list(),
+                              plan.chunk_train, plan.gseg, g_scale, mu, nu, count, count_done, lr, b1, b2, eps)
+            plan.adam_folded = True
+        else:
+            ops.backward(net.as_list(), theta, plan.dom, plan.x, wl.tensor_list(), wl.size_list(), plan.chunk_train,
+                         plan.gseg, plan.partial, g_scale)

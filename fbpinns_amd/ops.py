@@ -200,6 +200,30 @@ def pou_reduce(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], pairbu
                                 _p(ujet, torch.float32, "ujet")), "fbpx_pou_reduce")
 
 
+@torch.library.custom_op("fbpx::backward_adam", mutates_args=("theta", "mu", "nu", "count", "count_done"))
+def backward_adam(net: List[int], theta: torch.Tensor, dom: torch.Tensor, x: torch.Tensor, wl_t: List[torch.Tensor],
+                  wl_n: List[int], chunk_train: torch.Tensor, gseg: torch.Tensor, g_scale: Optional[torch.Tensor],
+                  mu: torch.Tensor, nu: torch.Tensor, count: torch.Tensor, count_done: torch.Tensor, lr: float,
+                  b1: float, b2: float, eps: float) -> None:
+    """fbpx_backward_adam: backward, then Adam on each trained subdomain row
+    from its one chunk row, in the backward's own launch (one-hidden-layer
+    networks, one chunk per trained row)."""
+    n = _net_from_list(net)
+    w = _wl_struct(wl_t, wl_n)
+    if g_scale is not None and g_scale.numel() != 1:
+        raise FbpxError(f"backward_adam: g_scale of {g_scale.numel()} values")
+    if mu.shape != theta.shape or nu.shape != theta.shape:
+        raise FbpxError(f"backward_adam: mu {tuple(mu.shape)} / nu {tuple(nu.shape)} for theta {tuple(theta.shape)}")
+    check(lib().fbpx_backward_adam(_stream(), ctypes.byref(n), _p(theta, torch.float32, "theta"),
+                                   _p(dom, torch.float32, "dom"), _p(x, torch.float32, "x"), ctypes.byref(w),
+                                   _p(chunk_train, torch.uint8, "chunk_train"), _p(gseg, torch.float32, "gseg"),
+                                   _p(g_scale, torch.float32, "g_scale"), _p(mu, torch.float32, "mu"),
+                                   _p(nu, torch.float32, "nu"), _p(count, torch.int32, "count"),
+                                   _p(count_done, torch.int32, "count_done"), float(lr), float(b1), float(b2),
+                                   float(eps)),
+          "fbpx_backward_adam")
+
+
 @torch.library.custom_op("fbpx::pou_backward", mutates_args=("gseg",))
 def pou_backward(net: List[int], wl_t: List[torch.Tensor], wl_n: List[int], wsum: torch.Tensor,
                  g_ujet: torch.Tensor, gseg: torch.Tensor, B: Optional[torch.Tensor] = None,
@@ -786,6 +810,12 @@ class Plan:
         # set by the engine's step: the backward leaves the per-chunk rows in
         # `partial` for adam_rows_partial and returns no theta gradient
         self.defer_rows = False
+        # set by the engine when every trained row is one chunk of a
+        # one-hidden-layer network: (theta, mu, nu, count, count_done, lr,
+        # b1, b2, eps) for a deferred-row backward to apply Adam itself
+        # (backward_adam); adam_folded records that it did, this step
+        self.adam_fold = None
+        self.adam_folded = False
 
     def keep_gradient_buffer(self, P):
         self.g_theta = torch.zeros((self.m, P), dtype=torch.float32, device=self.x.device)

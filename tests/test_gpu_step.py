@@ -74,7 +74,7 @@ CASES = {
 }
 
 
-def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True, dec_hi=None):
+def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True, dec_hi=None, fold_adam=True):
     """dec_hi: upper corner of the decomposition's box (default: the domain's)."""
     prob, kw, lo, hi, nsub, ov, ls, ns, sampler, ofac, has_mu = CASES[name]
     if ov is None:   # cfg 1: 15 centres, width 0.15
@@ -98,7 +98,7 @@ def _build(name, seed=0, graph=False, fused_loss=True, sort_points=True, dec_hi=
     constraints = [list(c[:-1]) for c in cg]
     req = [c[-1] for c in cg]
     eng = FusedEngine(prob, dec_cls, FCN, ap, optimiser_kwargs=dict(learning_rate=1e-3), device="cuda:0",
-                      graph=graph, fused_loss=fused_loss, sort_points=sort_points)
+                      graph=graph, fused_loss=fused_loss, sort_points=sort_points, fold_adam=fold_adam)
     eng.set_constraints(constraints, req)
     eng.set_active(np.ones(eng.m, dtype=int))
     return eng, ap, constraints, ofac, has_mu
@@ -339,6 +339,47 @@ def test_autotune_times_wave_bins_for_one_hidden_layer(monkeypatch):
     assert l1.item() == l0.item()
     assert ((g1 - g0).abs().max() / g0.abs().max()).item() < 1e-5
     autotune._cache.clear()
+
+
+@pytest.mark.parametrize("name,launch,graph", [("ho_hard_cfg1", None, False), ("ho_hard_cfg1", None, True),
+                                               ("ho_soft_two_constraints", None, True),
+                                               ("poisson_one_layer", (("waves", 2048),) * 2, False)])
+def test_adam_folded_into_backward_equals_separate_update(name, launch, graph):
+    """One hidden layer, no trainable problem leaf, every trained row one
+    chunk: the step applies Adam inside the backward's launch
+    (fbpx_backward_adam), and theta, both moments, the count and the losses
+    equal those of the backward followed by fbpx_adam_rows_partial, bit for
+    bit, over several steps (eager and graph-replayed; workgroup and wave
+    bins)."""
+    ea, *_ = _build(name, graph=graph, fold_adam=False)
+    eb, *_ = _build(name, graph=graph, fold_adam=True)
+    if launch is not None:
+        for e in (ea, eb):
+            e.set_active(np.ones(e.m, dtype=int), launch=launch)
+    assert ea.plan.adam_fold is None and eb.plan.adam_fold is not None
+    print(f"[adam fold] {name} graph={graph} launch={eb.launch} chunks {eb.plan.wl.n_chunks} "
+          f"rows {eb.plan.rows.numel()}", flush=True)
+    for k in range(3):
+        la, lb = ea.step(), eb.step()
+        torch.cuda.synchronize()
+        assert eb.plan.adam_folded
+        assert la.item() == lb.item(), k
+        for a, b in ((ea.theta, eb.theta), (ea.mu, eb.mu), (ea.nu, eb.nu), (ea.count, eb.count)):
+            assert torch.equal(a, b), k
+    assert int(eb.count.item()) == 3 and int(eb.count_done.item()) == 0
+
+
+def test_adam_fold_only_where_each_trained_row_is_one_chunk():
+    eng, *_ = _build("ho_inverse_mu")            # a trainable problem leaf
+    assert eng.plan.adam_fold is None
+    eng, *_ = _build("ho_nonaffine_generic")     # two hidden layers
+    assert eng.plan.adam_fold is None
+    eng, *_ = _build("ho_hard_cfg1")
+    eng.set_active(np.ones(eng.m, dtype=int), chunk=8)   # several chunks per row
+    assert eng.plan.adam_fold is None
+    eng.step()
+    torch.cuda.synchronize()
+    assert not eng.plan.adam_folded and int(eng.count.item()) == 1
 
 
 def test_wave_bins_rejected_for_two_hidden_layers():

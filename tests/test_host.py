@@ -446,6 +446,14 @@ def test_library_shape_queries_and_error_paths():
         rc = L.fbpx_forward(None, ctypes.byref(n), None, None, None, ctypes.byref(wl), None)
         assert rc == -1 and msg in L.fbpx_last_error().decode(), (nh, fw, bw)
     assert ctypes.sizeof(_lib.fbpx_worklist) == 7 * 4 + 4 + 9 * 8 + 2 * 4
+    # the folded optimizer step: one hidden layer, every optimizer pointer given
+    wl = _lib.fbpx_worklist()
+    fake = ctypes.c_void_p(0x1000)
+    for nh, mu, msg in ((2, fake, "one-hidden-layer"), (1, None, "null mu")):
+        n = _lib.fbpx_net(1, 32, nh, 1, 1)
+        rc = L.fbpx_backward_adam(None, ctypes.byref(n), fake, fake, fake, ctypes.byref(wl), None, fake, None, mu,
+                                  fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8)
+        assert rc == -1 and msg in L.fbpx_last_error().decode(), nh
 
 
 def test_pou_loss_and_residual_grad_validate_arguments():
