@@ -241,11 +241,13 @@ int fbpx_backward(void* stream, const fbpx_net* net, const float* theta,
  * finishes a chunk applies optax Adam (as fbpx_adam_rows, lr, b1, b2, eps)
  * to its subdomain's row of theta, mu and nu in place, with the chunk's
  * gradient row as the row's gradient, instead of writing a partial row, and
- * the last workgroup adds 1 to count (count_done: int32 0, left 0).  theta,
- * mu, nu, count and gseg-derived rows equal fbpx_backward followed by
- * fbpx_adam_rows_partial bit for bit.  Untrained chunks (chunk_train 0) are
- * left alone.  No counterpart in the reference beyond those two calls' (one
- * launch fewer in a step of a few microseconds). */
+ * the last workgroup (or wave, with bwd_wave_bins) to finish adds 1 to count
+ * (count_done: int32 0, left 0).  theta, mu, nu and count equal
+ * fbpx_backward followed by fbpx_adam_rows_partial bit for bit.  Untrained
+ * chunks (chunk_train 0) are left alone.  Replaces what those two calls
+ * replace: the reverse mode of value_and_grad(FBPINN_loss) below the
+ * segment sums and optax.adam's update (fbpinns/trainers.py:292-295), in
+ * one launch instead of two. */
 int fbpx_backward_adam(void* stream, const fbpx_net* net, float* theta, const float* dom, const float* x,
                        const fbpx_worklist* wl, const uint8_t* chunk_train, const float* gseg, const float* g_scale,
                        float* mu, float* nu, int32_t* count, int32_t* count_done, double lr, double b1, double b2,
