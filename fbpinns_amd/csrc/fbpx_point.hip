@@ -905,6 +905,7 @@ __device__ inline float sel(const float* v, int c) {
 // same float expressions as the separate launches use.
 struct PouLossArgs {
   int n_res;
+  int mem[FBPX_POU_LOSS_RES];       // bit f: column f is a memory column (f < n_cols, jet[f] < 0)
   fbpx_residual res[FBPX_POU_LOSS_RES];
   float scale[FBPX_POU_LOSS_RES];   // weight / n
   float w2n[FBPX_POU_LOSS_RES];     // 2 weight / n
@@ -959,17 +960,35 @@ __global__ void __launch_bounds__(256) pou_loss_kernel(fbpx_worklist wl, const f
       Aj[c] = A ? A[(size_t)c * wl.n_points + p] : 0.f;
       Bj[c] = B ? B[(size_t)c * wl.n_points + p] : 0.f;
     }
+    // every gathered column's index first, then every value: two loads'
+    // latency in a thread's chain, not two per column
     float M[NR][NC];
+    int kc[NR][NC];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
+      const int mi = a.mem[i];   // 0 past n_res
+#pragma unroll
+      for (int f = 0; f < NC; ++f) M[i][f] = 0.f;
+      if (mi == 0) continue;
       const fbpx_residual& R = a.res[i];
       const int k = p - R.offset;
-      const bool on = i < a.n_res && k >= 0 && k < R.n;
+      const bool on = k >= 0 && k < R.n;
 #pragma unroll
       for (int f = 0; f < NC; ++f) {
-        M[i][f] = 0.f;
-        if (on && f < R.n_cols && R.jet[f] < 0) M[i][f] = R.col[f][R.col_idx[f] ? R.col_idx[f][k] : k];
+        kc[i][f] = k;
+        if (on && ((mi >> f) & 1) && R.col_idx[f]) kc[i][f] = R.col_idx[f][k];
       }
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int mi = a.mem[i];
+      if (mi == 0) continue;
+      const fbpx_residual& R = a.res[i];
+      const int k = p - R.offset;
+      const bool on = k >= 0 && k < R.n;
+#pragma unroll
+      for (int f = 0; f < NC; ++f)
+        if (on && ((mi >> f) & 1)) M[i][f] = R.col[f][kc[i][f]];
     }
     // the POU reduction (pou_point without its affine loads)
     float u[J], g[J];
@@ -1599,6 +1618,7 @@ int fbpx_pou_loss(void* stream, const fbpx_net* net, const fbpx_worklist* wl, co
     }
     a.res[i] = R;
     for (int f = R.n_cols; f < FBPX_POU_LOSS_COLS; ++f) a.res[i].jet[f] = -1;
+    for (int f = 0; f < R.n_cols; ++f) a.mem[i] |= (R.jet[f] < 0) << f;
     for (int t = 0; t < R.n_terms; ++t) {
       a.res[i].fa[t] = R.fa[t] < 0 ? -1 : R.fa[t];
       a.res[i].fb[t] = R.fb[t] < 0 ? -1 : R.fb[t];
