@@ -524,13 +524,20 @@ __global__ void __launch_bounds__(256) sum_stage2_kernel(const float* __restrict
 // ------------------------------------------------------------------ fused residual loss
 
 // Column and term tables of one residual (factor index -1: the factor 1).
-// cidx[f] != NULL: column f is gathered, F(f, k) = col[f][cidx[f][k]].
+// Term t's factors are read through its own column pointers, filled on the
+// host from the column table (col[fa[t]], col[fb[t]]), so that a workgroup
+// loads them all at once instead of a factor index and then the pointer it
+// selects.  ia/ib[t] != NULL: that factor's column f is gathered, F(f, k) =
+// col[f][col_idx[f][k]].
 struct ResidualArgs {
   int n_cols, n_terms;
-  const float* col[FBPX_MAX_REQ];
-  const int32_t* cidx[FBPX_MAX_REQ];
+  const float* pa[FBPX_MAX_TERMS];
+  const float* pb[FBPX_MAX_TERMS];
+  const int32_t* ia[FBPX_MAX_TERMS];
+  const int32_t* ib[FBPX_MAX_TERMS];
   float* gcol[FBPX_MAX_REQ];
   int fa[FBPX_MAX_TERMS], fb[FBPX_MAX_TERMS];
+  int gather;   // bit t: ia[t] != NULL; bit 8 + t: ib[t] != NULL
 };
 
 // Points per thread and loop trip: independent loads in flight per thread.
@@ -569,9 +576,15 @@ __device__ inline void last_block_sum(const float* scratch, int ncol, float scal
   if (threadIdx.x == 0) atomicExch(done, 0);
 }
 
+// The index 0 that a factor without a gather index reads, so that the
+// index loads of a trip need no branch per factor.
+__device__ int32_t kResIdx0[1] = {0};
+
 // Term t's factor columns, index masks (all-ones for a column, 0 for the
-// factor 1, which reads kResOne[0]) and gather indices (NULL: none), uniform
-// across the grid.
+// factor 1, which reads kResOne[0]) and gather indices (ia/ib[t] read at
+// point k & ja/jb[t]: the column's index, or kResIdx0[0] with mask 0 where
+// the factor is not gathered), uniform across the grid.  gather: bit t (8 +
+// t) where factor a (b) of term t is gathered.
 template <int NT>
 struct TermCols {
   const float* pa[NT];
@@ -579,12 +592,47 @@ struct TermCols {
   int64_t ma[NT], mb[NT];
   const int32_t* ia[NT];
   const int32_t* ib[NT];
+  int64_t ja[NT], jb[NT];
+  int gather;
 };
 
-// Factor value at point k: p[k], p[idx[k]] for a gathered column, p[0] for
-// the factor 1 (mask 0).
-__device__ inline float res_factor(const float* p, const int32_t* idx, int64_t k, int64_t mask) {
-  return p[(idx ? (int64_t)idx[k] : k) & mask];
+// The factors of one loop trip, xa/xb[u][t] at the points k = k0 + u *
+// stride (past n: point 0, the result dropped): p[k], p[idx[k]] for a
+// gathered column, p[0] for the factor 1 (mask 0).  When any factor is
+// gathered, every index of the trip is loaded, unconditionally, before any
+// value, so that a thread waits on two round trips, not on two per gathered
+// factor (a load behind a branch of its own is waited for inside it).
+template <int NT>
+__device__ inline void res_factors(const TermCols<NT>& c, int64_t k0, int64_t stride, int64_t n,
+                                   float (*xa)[NT], float (*xb)[NT]) {
+  int32_t ga[kResUnroll][NT], gb[kResUnroll][NT];
+  if (c.gather != 0) {
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u) {
+      const int64_t k = k0 + u * stride;
+      const int64_t kk = k < n ? k : 0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        ga[u][t] = c.ia[t][kk & c.ja[t]];
+        gb[u][t] = c.ib[t][kk & c.jb[t]];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kResUnroll; ++u)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) ga[u][t] = gb[u][t] = 0;
+  }
+#pragma unroll
+  for (int u = 0; u < kResUnroll; ++u) {
+    const int64_t k = k0 + u * stride;
+    const int64_t kk = k < n ? k : 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      xa[u][t] = c.pa[t][(((c.gather >> t) & 1) ? (int64_t)ga[u][t] : kk) & c.ma[t]];
+      xb[u][t] = c.pb[t][(((c.gather >> (8 + t)) & 1) ? (int64_t)gb[u][t] : kk) & c.mb[t]];
+    }
+  }
 }
 
 static_assert(FBPX_MAX_TERMS == 8, "the residual launches below instantiate 1 .. 8 terms");
@@ -595,13 +643,17 @@ __device__ inline TermCols<NT> term_cols(const ResidualArgs& a) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const bool ha = a.fa[t] >= 0, hb = a.fb[t] >= 0;
-    c.pa[t] = ha ? a.col[a.fa[t]] : kResOne;
-    c.pb[t] = hb ? a.col[a.fb[t]] : kResOne;
+    c.pa[t] = ha ? a.pa[t] : kResOne;
+    c.pb[t] = hb ? a.pb[t] : kResOne;
     c.ma[t] = ha ? ~int64_t(0) : 0;
     c.mb[t] = hb ? ~int64_t(0) : 0;
-    c.ia[t] = ha ? a.cidx[a.fa[t]] : nullptr;
-    c.ib[t] = hb ? a.cidx[a.fb[t]] : nullptr;
+    const bool ga = (a.gather >> t) & 1, gb = (a.gather >> (8 + t)) & 1;
+    c.ia[t] = ga ? a.ia[t] : kResIdx0;
+    c.ib[t] = gb ? a.ib[t] : kResIdx0;
+    c.ja[t] = ga ? ~int64_t(0) : 0;
+    c.jb[t] = gb ? ~int64_t(0) : 0;
   }
+  c.gather = a.gather;
   return c;
 }
 
@@ -647,16 +699,7 @@ __device__ inline float residual_block(int64_t n, const ResidualArgs& a, const f
   float s = 0.f;
   for (int64_t k0 = lb * blockDim.x + threadIdx.x; k0 < n; k0 += kResUnroll * stride) {
     float xa[kResUnroll][NT], xb[kResUnroll][NT], sr[kResUnroll];
-#pragma unroll
-    for (int u = 0; u < kResUnroll; ++u) {
-      const int64_t k = k0 + u * stride;
-      const int64_t kk = k < n ? k : 0;   // past the end: read point 0, result dropped
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        xa[u][t] = res_factor(c.pa[t], c.ia[t], kk, c.ma[t]);
-        xb[u][t] = res_factor(c.pb[t], c.ib[t], kk, c.mb[t]);
-      }
-    }
+    res_factors<NT>(c, k0, stride, n, xa, xb);
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
@@ -802,12 +845,8 @@ __global__ void __launch_bounds__(256) residual_backward_kernel(int64_t n, Resid
       const int64_t k = k0 + u * stride;
       const int64_t kk = k < n ? k : 0;
       sr[u] = r[kk];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        xa[u][t] = res_factor(c.pa[t], c.ia[t], kk, c.ma[t]);
-        xb[u][t] = res_factor(c.pb[t], c.ib[t], kk, c.mb[t]);
-      }
     }
+    res_factors<NT>(c, k0, stride, n, xa, xb);
 #pragma unroll
     for (int u = 0; u < kResUnroll; ++u) {
       const int64_t k = k0 + u * stride;
@@ -1404,8 +1443,6 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, con
       set_error("residual loss: null column %d", f);
       return FBPX_EINVAL;
     }
-    a->col[f] = col[f];
-    a->cidx[f] = col_idx ? col_idx[f] : nullptr;
     a->gcol[f] = gcol ? gcol[f] : nullptr;
   }
   for (int t = 0; t < n_terms; ++t) {
@@ -1415,6 +1452,15 @@ static int residual_args(int64_t n, int32_t n_cols, const float* const* col, con
     }
     a->fa[t] = fa[t] < 0 ? -1 : fa[t];
     a->fb[t] = fb[t] < 0 ? -1 : fb[t];
+    if (a->fa[t] >= 0) {
+      a->pa[t] = col[a->fa[t]];
+      a->ia[t] = col_idx ? col_idx[a->fa[t]] : nullptr;
+    }
+    if (a->fb[t] >= 0) {
+      a->pb[t] = col[a->fb[t]];
+      a->ib[t] = col_idx ? col_idx[a->fb[t]] : nullptr;
+    }
+    a->gather |= (a->ia[t] != nullptr) << t | (a->ib[t] != nullptr) << (8 + t);
   }
   return FBPX_OK;
 }
