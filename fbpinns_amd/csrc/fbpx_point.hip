@@ -427,6 +427,10 @@ __device__ inline void adam_count_step(int32_t* count, int32_t* count_done) {
 // PARTIAL false: the gradient rows grad; true: the per-chunk partial rows,
 // summed per listed row as fbpx_reduce_rows sums them (ascending chunks,
 // same unrolling), so that both forms give the same update bit for bit.
+// One element per thread: the launch covers n_rows * P (n_rows >= 1).  The
+// first wave forms the bias corrections while its first loads are in
+// flight, so that the count's round trip and the powf do not come before
+// them.
 template <bool PARTIAL>
 __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ theta, float* __restrict__ mu,
                                                         float* __restrict__ nu, int P,
@@ -435,30 +439,39 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
                                                         const int32_t* __restrict__ rcs,
                                                         const int32_t* __restrict__ rows, int n_rows,
                                                         int32_t* count, int32_t* count_done, AdamConst k) {
-  // bias corrections once per workgroup
   __shared__ float sbc[2];
-  if (threadIdx.x == 0) adam_bias(count, k, sbc);
-  __syncthreads();
-  const float bc1 = sbc[0], bc2 = sbc[1];
-  const int64_t total = (int64_t)n_rows * P;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int row = (int)(idx / P), e = (int)(idx % P);
-    // two dependent round trips: the row and its chunk range, then the
-    // parameter, its moments and the gradient (or the chunk rows) together
-    const int r = rows[row];
-    const int c0 = PARTIAL ? rcs[row] : 0, c1 = PARTIAL ? rcs[row + 1] : 0;
-    const size_t i = (size_t)r * P + e;
-    float p = theta[i], m = mu[i], v = nu[i];
-    float gi;
+  const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);   // n_rows * P < 2^31 - 255
+  const bool on = idx < n_rows * P;
+  // two dependent round trips: the row and its chunk range (past the end:
+  // row 0's, unused), then the parameter, its moments and the gradient (or
+  // the chunk rows) together
+  const int row = on ? idx / P : 0, e = on ? idx - row * P : 0;
+  const int r = rows[row];
+  const int c0 = PARTIAL ? rcs[row] : 0, c1 = PARTIAL ? rcs[row + 1] : 0;
+  if (threadIdx.x < 64) {
+    float bc[2];
+    adam_bias(count, k, bc);
+    if (threadIdx.x == 0) {
+      sbc[0] = bc[0];
+      sbc[1] = bc[1];
+    }
+  }
+  const size_t i = (size_t)r * P + e;
+  float p = 0.f, m = 0.f, v = 0.f, gi = 0.f;
+  if (on) {
+    p = theta[i];
+    m = mu[i];
+    v = nu[i];
     if constexpr (!PARTIAL) {
       gi = grad[i];
     } else {
-      gi = 0.f;
 #pragma unroll 4
       for (int c = c0; c < c1; ++c) gi += partial[(size_t)c * P + e];
     }
-    adam_elem(p, m, v, gi, bc1, bc2, k);
+  }
+  __syncthreads();
+  if (on) {
+    adam_elem(p, m, v, gi, sbc[0], sbc[1], k);
     theta[i] = p;
     mu[i] = m;
     nu[i] = v;
@@ -466,16 +479,23 @@ __global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ thet
   adam_count_step(count, count_done);
 }
 
+// As adam_rows_kernel, over a flat vector: one element per thread.
 __global__ void __launch_bounds__(256) adam_flat_kernel(float* __restrict__ p, float* __restrict__ mu,
                                                         float* __restrict__ nu, const float* __restrict__ g, int n,
                                                         int32_t* count, int32_t* count_done, AdamConst k) {
   __shared__ float sbc[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float pv = 0.f, m = 0.f, v = 0.f, gv = 0.f;
+  if (i < n) {
+    pv = p[i];
+    m = mu[i];
+    v = nu[i];
+    gv = g[i];
+  }
   if (threadIdx.x == 0) adam_bias(count, k, sbc);
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    float pv = p[i], m = mu[i], v = nu[i];
-    adam_elem(pv, m, v, g[i], sbc[0], sbc[1], k);
+    adam_elem(pv, m, v, gv, sbc[0], sbc[1], k);
     p[i] = pv;
     mu[i] = m;
     nu[i] = v;
@@ -1262,8 +1282,8 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial, const int32_
 static int adam_rows_launch(void* stream, float* theta, float* mu, float* nu, int32_t P, const float* grad,
                             const float* partial, const int32_t* rcs, const int32_t* rows, int32_t n_rows,
                             int32_t* count, int32_t* count_done, double lr, double b1, double b2, double eps) {
-  if (P <= 0 || n_rows < 0) {
-    set_error("adam_rows: bad sizes P=%d n_rows=%d", P, n_rows);
+  if (P <= 0 || n_rows < 0 || (int64_t)n_rows * P > INT32_MAX - 255) {
+    set_error("adam_rows: bad sizes P=%d n_rows=%d (n_rows * P below 2^31 - 255)", P, n_rows);
     return FBPX_EINVAL;
   }
   if (!count) {
