@@ -221,11 +221,15 @@ struct AdamFold {
   AdamConst k;
 };
 
-// bc[0] = 1 - b1^c, bc[1] = 1 - b2^c for c = count + 1
-__device__ inline void adam_bias(const int32_t* count, const AdamConst& k, float* bc) {
-  const float cf = (float)(count[0] + 1);
+// bc[0] = 1 - b1^c, bc[1] = 1 - b2^c for c = n + 1, n the step count
+__device__ inline void adam_bias_n(int32_t n, const AdamConst& k, float* bc) {
+  const float cf = (float)(n + 1);
   bc[0] = 1.f - powf(k.b1, cf);
   bc[1] = 1.f - powf(k.b2, cf);
+}
+
+__device__ inline void adam_bias(const int32_t* count, const AdamConst& k, float* bc) {
+  adam_bias_n(count[0], k, bc);
 }
 
 __device__ inline void adam_elem(float& p, float& m, float& v, float g, float bc1, float bc2, const AdamConst& k) {

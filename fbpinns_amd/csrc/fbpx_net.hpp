@@ -757,10 +757,12 @@ __device__ inline void finish_row(float* sacc, const float* ssmall, const float*
 // [xd, h, 1]: no hidden-to-hidden layer; waves work independently.
 // WAVE: the chunk is one wave's (its own LDS copies, its slices of the
 // per-wave sums).
+// ADAM: bc holds the bias corrections, formed by the team's first lane in
+// its first chunk (first) while the chunk's domain loads are in flight.
 template <int XD, int H, int D1M, int D2M, bool WAVE = false, bool ADAM = false>
 __device__ inline void bwd_one(const float* theta, const float* dom, const float* x, const fbpx_worklist& wl,
                                const float* gseg, float* partial, int chunk, float gs, const AdamFold& af,
-                               const float* bc) {
+                               float* bc, bool first = false) {
   using S = Schema<XD, D1M, D2M>;
   using L = Layout<XD, H, 1>;
   using SM = Small<XD, H, 1>;
@@ -781,6 +783,8 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   const int model = wl.chunk_model[chunk];
   const int beg = wl.chunk_start[chunk];
   const int end = wl.chunk_start[chunk + 1];
+  int32_t cnt = 0;
+  if constexpr (ADAM) cnt = af.count[0];   // issued with the chunk's first loads
   const float* th = theta + (size_t)model * L::P;
   const int lane = threadIdx.x & 63, wave = T::wave(), nw = T::nw();
   const int pl = lane & 15, g = lane >> 4;
@@ -799,6 +803,9 @@ __device__ inline void bwd_one(const float* theta, const float* dom, const float
   if (T::tid() < T::nw() * 16) sbl[T::tid()] = 0.f;
   Dom d;
   load_dom(dom, model, d);
+  if constexpr (ADAM) {
+    if (first && T::tid() == 0) adam_bias_n(cnt, af.k, bc);
+  }
   float mu[XD], isd[XD];
 #pragma unroll
   for (int i = 0; i < XD; ++i) {
@@ -1439,14 +1446,12 @@ bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
   static_assert(!WAVE || NH == 1, "one wave per chunk: one-hidden-layer networks");
   static_assert(!ADAM || NH == 1, "the folded optimizer step: one-hidden-layer networks");
   const float gs = g_scale ? g_scale[0] : 1.f;
-  // the bias corrections, from the count as every team reads it before the
-  // last one advances it (the barrier completes the read)
+  // the bias corrections, formed in LDS by each team in its first chunk from
+  // the count as it reads it before the last team advances it (the read
+  // completes before the team's own count below), read where a row is
+  // applied (after the chunk's barriers)
   __shared__ float sbc[2];
-  if constexpr (ADAM) {
-    if (threadIdx.x == 0) adam_bias(af.count, af.k, sbc);
-    __syncthreads();
-  }
-  const float* const bc = sbc;   // read in LDS where a row is applied
+  bool first = true;
   // persistent: workgroup b runs the chunks of backward bin b in order (WAVE:
   // wave w runs bin 4b + w)
   const int bin = WAVE ? (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6) : (int)blockIdx.x;
@@ -1456,9 +1461,10 @@ bwd_kernel(const float* __restrict__ theta, const float* __restrict__ dom, const
     if (chunk_train != nullptr && chunk_train[chunk] == 0) continue;
     Team<WAVE>::sync();   // the team is done with the previous chunk's LDS
     if constexpr (NH == 1)
-      bwd_one<XD, H, D1M, D2M, WAVE, ADAM>(theta, dom, x, wl, gseg, partial, chunk, gs, af, bc);
+      bwd_one<XD, H, D1M, D2M, WAVE, ADAM>(theta, dom, x, wl, gseg, partial, chunk, gs, af, sbc, first);
     else
-      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs, af, bc);
+      bwd_two<XD, H, D1M, D2M>(theta, dom, x, wl, gseg, partial, chunk, gs, af, sbc);
+    first = false;
   }
   if constexpr (ADAM) {
     if (Team<WAVE>::tid() == 0 && atomicAdd(af.count_done, 1) == wl.n_bbins - 1) {

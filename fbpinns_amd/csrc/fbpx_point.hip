@@ -283,8 +283,23 @@ __global__ void __launch_bounds__(256) pou_reduce_kernel(fbpx_worklist wl, const
   constexpr int J = S::J;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= wl.n_points) return;
+  // the affine jets do not depend on the reduction: loaded ahead of its
+  // chain of dependent loads, not after it (pou_point's own affine step,
+  // with the same arithmetic)
+  float Aj[J], Bj[J];
+#pragma unroll
+  for (int c = 0; c < J; ++c) {
+    Aj[c] = A ? A[(size_t)c * wl.n_points + p] : 0.f;
+    Bj[c] = B ? B[(size_t)c * wl.n_points + p] : 0.f;
+  }
   float acc[J];
-  pou_point<S>(wl, pairbuf, wsum, A, B, p, acc);
+  pou_point<S>(wl, pairbuf, wsum, nullptr, nullptr, p, acc);
+  if (A != nullptr) {
+    float q[J];
+#pragma unroll
+    for (int c = 0; c < J; ++c) q[c] = acc[c];
+    leibniz<S>(Aj, Bj, q, acc);
+  }
 #pragma unroll
   for (int c = 0; c < J; ++c) ujet[(size_t)c * wl.n_points + p] = acc[c];
 }
