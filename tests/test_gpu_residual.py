@@ -13,6 +13,10 @@ transpose, ops.ResidualLoss) against the problems' own loss_fn.
 * the per-point gather of a trainable vector and its transpose
   (fbpx_segment_sum) against float64 index_add, with unsorted indices and
   empty segments;
+* two gathered columns with index vectors of their own, read as a term's
+  second factor, as both factors of one term and beside plain columns: the
+  forward with its column gradients and the backward with the coefficient
+  gradients against float64;
 * fbpx_residual_loss_grad (the loss launch that also writes the column
   gradients at unit upstream gradient) against fbpx_residual_loss followed
   by fbpx_residual_loss_backward with g_loss = 1, bit for bit, and
@@ -218,6 +222,61 @@ def test_single_launch_sum_equals_two_launch_sum(n):
     print(f"[residual] two launches {out[0]!r} one launch {out[1]!r} float64 {ref!r}", flush=True)
     assert out[0] == out[1]
     assert abs(out[0] - ref) / ref < 1e-5
+
+
+@pytest.mark.parametrize("n,n_vec", [(1000, 7), (70_001, 3), (300_000, 256)])
+def test_gathered_factors_in_every_position(n, n_vec):
+    """Two columns gathered through two index vectors, read as a term's
+    second factor, as both factors of one term and beside a plain column:
+    r = c0 a v[i] + c1 v[i]^2 + c2 w[j] + c3 b w[j].  The forward with its
+    column gradients (fbpx_residual_loss_grad) and the backward with the
+    coefficient gradients (fbpx_residual_loss_backward) against float64
+    torch: r and the loss 1e-5 relative, the gradients 1e-4."""
+    from fbpinns_amd import _lib, ops
+    dev = torch.device("cuda")
+    g = torch.Generator().manual_seed(int(n) % 89 + 5)
+    n_w = max(1, n_vec // 3)
+    a, b = torch.randn(n, generator=g).to(dev), torch.randn(n, generator=g).to(dev)
+    v, w = torch.randn(n_vec, generator=g).to(dev), torch.randn(n_w, generator=g).to(dev)
+    iv = torch.randint(0, n_vec, (n,), generator=g).to(dev, torch.int32)
+    iw = torch.randint(0, n_w, (n,), generator=g).to(dev, torch.int32)
+    coef = torch.tensor([0.7, -1.1, 0.4, 1.6], device=dev)
+    fa, fb = [0, 1, 2, 3], [1, 1, -1, 2]
+    cols, weight = [a, v, w, b], 0.8
+    print(f"[gather positions] n={n} n_vec={n_vec} n_w={n_w}", flush=True)
+    r = torch.full((n,), float("nan"), device=dev)
+    loss = torch.empty((), device=dev)
+    done = torch.zeros(1, dtype=torch.int32, device=dev)
+    gf = [torch.full((n,), float("nan"), device=dev) for _ in range(4)]
+    scratch = torch.empty(_lib.FBPX_RESIDUAL_BLOCKS * 4, device=dev)
+    ops.residual_loss_grad(cols, fa, fb, coef, weight, r, scratch, loss, done, [1, 2], [iv, iw], False,
+                           [0, 1, 2, 3], gf)
+    g_loss = torch.tensor([1.3], device=dev)
+    gb = [torch.full((n,), float("nan"), device=dev) for _ in range(4)]
+    gcoef = torch.full((4,), float("nan"), device=dev)
+    ops.residual_loss_backward(cols, fa, fb, coef, r, weight, g_loss, [0, 1, 2, 3], gb, scratch, gcoef, True,
+                               done, [1, 2], [iv, iw])
+    torch.cuda.synchronize()
+    a64, b64 = a.double().cpu(), b.double().cpu()
+    vi, wj = v.double().cpu()[iv.long().cpu()], w.double().cpu()[iw.long().cpu()]
+    c = coef.double().cpu()
+    terms = [c[0] * a64 * vi, c[1] * vi * vi, c[2] * wj, c[3] * b64 * wj]
+    rr = sum(terms)
+    lref = weight * (rr ** 2).mean().item()
+    w2n = 2.0 * weight / n
+    dcol = [c[0] * vi, c[0] * a64 + 2.0 * c[1] * vi, c[2] + c[3] * b64, c[3] * wj]
+    assert torch.allclose(r.double().cpu(), rr, rtol=1e-5, atol=1e-5)
+    assert abs(loss.item() - lref) <= 1e-5 * abs(lref)
+    for f in range(4):
+        for scale, got in ((1.0, gf[f]), (1.3, gb[f])):
+            ref = scale * w2n * rr * dcol[f]
+            e = ((got.double().cpu() - ref).abs().max() / ref.abs().max()).item()
+            assert e <= 1e-4, (f, scale, e)
+    cref = torch.stack([1.3 * w2n * (rr * t / c[k]).sum() for k, t in enumerate(terms)])
+    e = ((gcoef.double().cpu() - cref).abs().max() / cref.abs().max()).item()
+    print(f"  loss {loss.item():.9e} float64 {lref:.9e}; coefficient gradients rel {e:.1e}", flush=True)
+    assert e <= 1e-4
+    assert int(done.item()) == 0
 
 
 @pytest.mark.parametrize("n,n_vec,use_done", [(1000, 7, True), (300_000, 256, False), (300_000, 256, True)])
