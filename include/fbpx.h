@@ -275,6 +275,7 @@ int fbpx_reduce_rows(void* stream, int32_t P, const float* partial,
  *   mu = (1-b1) g + b1 mu;  nu = (1-b2) g^2 + b2 nu;
  *   p += -lr * (mu/(1-b1^c)) / (sqrt(nu/(1-b2^c)) + eps),  c = *count + 1.
  * Rows not listed (fixed or discarded subdomains) are untouched.
+ * n_rows * P < 2^31 - 255 (FBPX_EINVAL otherwise).
  * theta, mu, nu, grad: [m][P].  The hyper-parameters are doubles so that
  * f32(1 - b1) and f32(-lr) are formed as the reference forms them (Python
  * floats meeting FP32 arrays).  count_done: NULL, or a DEVICE int32 that is 0

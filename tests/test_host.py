@@ -454,6 +454,11 @@ def test_library_shape_queries_and_error_paths():
         rc = L.fbpx_backward_adam(None, ctypes.byref(n), fake, fake, fake, ctypes.byref(wl), None, fake, None, mu,
                                   fake, fake, fake, 1e-3, 0.9, 0.999, 1e-8)
         assert rc == -1 and msg in L.fbpx_last_error().decode(), nh
+    # Adam over rows: one 32-bit element index per thread, n_rows * P < 2^31 - 255
+    cnt = ctypes.c_void_p(0x2000)
+    for P, n_rows in ((97, (2 ** 31 - 255) // 97 + 1), (1 << 16, 1 << 15)):
+        rc = L.fbpx_adam_rows(None, fake, fake, fake, P, fake, fake, n_rows, cnt, None, 1e-3, 0.9, 0.999, 1e-8)
+        assert rc == -1 and "bad sizes" in L.fbpx_last_error().decode(), (P, n_rows)
 
 
 def test_pou_loss_and_residual_grad_validate_arguments():
